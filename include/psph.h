@@ -1,0 +1,250 @@
+/*
+ * psph.h -- C ABI of the MI355X per-step SPH particle path (Plume-SPH drop-in).
+ *
+ * The reference has no plugin or FFI layer: its hot-path stages are C++ free
+ * functions that mutate Particle objects through THashTable and HashTable pointers
+ * (src/plume/particler.h:22-357), called only from the time loop in
+ * src/plume/main.cc:217-486 (3D) and main.cc:676-751 (1D).  Each entry point
+ * below replaces one of those functions and cites it.  Plain pointers and
+ * sizes only; no torch or HIP types cross this boundary.
+ *
+ * Ownership: the caller owns every host buffer (copied in/out).  A psph_ctx
+ * owns all device memory and one HIP stream.  Calls on one context are
+ * serialised and stream-ordered; contexts are independent.
+ *
+ * Errors: every function returns 0 on success or a negative PSPH_E* code; the
+ * reference aborts (assert/exit/MPI_Abort) where this returns a code.  The
+ * message of the last failure on the calling thread is psph_last_error().
+ */
+#ifndef PSPH_H
+#define PSPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PSPH_OK           0
+#define PSPH_EINVAL      -1   /* bad argument or configuration                */
+#define PSPH_ENOMEM      -2   /* device or host allocation failed             */
+#define PSPH_EINVARIANT  -3   /* rho<=0, c<=0, xi>1, wnorm<=0 ... (asserts)   */
+#define PSPH_ECAPACITY   -4   /* particle capacity / inject buffer overflow   */
+#define PSPH_EHIP        -5   /* HIP runtime error                            */
+#define PSPH_EIO         -6   /* deck / meteo file cannot be read             */
+#define PSPH_ENOTSUP     -7   /* option not implemented                       */
+
+/* ---- particle boundary-condition types (particle.h:157-166) ------------- */
+#define PSPH_BC_ERUPT_GHOST     0
+#define PSPH_BC_PRESSURE_GHOST  1
+#define PSPH_BC_WALL_GHOST      2
+#define PSPH_BC_NOSAVE_REAL    99
+#define PSPH_BC_REAL          100
+
+/* ---- precision forms --------------------------------------------------- */
+#define PSPH_FP64        0   /* plain FP64 (drop-in parity form)              */
+#define PSPH_FP32_PAIR   1   /* FP32 cell-relative pair sums, FP64 elsewhere  */
+
+/* ---- smoothing kernels ------------------------------------------------- */
+#define PSPH_KERNEL_GAUSS_BOX   0  /* sph_lib.cc:207-253 (the reference's)    */
+#define PSPH_KERNEL_CUBIC_M4    1  /* Monaghan 1992 M4 spline, support 2h     */
+
+/*
+ * Solver configuration: every compile-time switch (options.h), constant
+ * (constant.h) and physical parameter (parameters.h) the hot path reads,
+ * as runtime values.  psph_config_default() fills the reference defaults.
+ */
+typedef struct psph_config {
+  int32_t dim;              /* CODE_DIMENSION: 1 or 3                          */
+  int32_t precision;        /* PSPH_FP64 / PSPH_FP32_PAIR                      */
+  int32_t kernel;           /* PSPH_KERNEL_*                                   */
+  int32_t lans;             /* HAVE_TURBULENCE_LANS (0 or 1)                   */
+  int32_t energy_smooth;    /* HAVE_ENERGY_SMOOTH (0 or 1)                     */
+  int32_t pghost_contrib;   /* PGHOST_CONTRIBUTE_DES (0, 1 or 2)               */
+  int32_t list_all;         /* 1: every particle lists every particle (1D)     */
+  int32_t num_loop;         /* num_loop_P: adaptive-h iterations               */
+  int32_t sml_update_int;   /* SML_UPDATE_INT: adaptive-h cadence (0 = off)    */
+  int32_t adapt_at_setup;   /* 1D main.cc:641: adaptive_sml once before loop   */
+  int32_t vent_profile;     /* ERUPT_VELOCITY_PROF: 0 uniform, 1 parabolic     */
+  int32_t max_steps;        /* deck line 2                                     */
+  int32_t num_erupt_particles; /* parameters.h:983                            */
+  int32_t p_table_size;     /* P_TABLE_SIZE: slots of the particle hash table  */
+  int32_t reserved_i[2];
+
+  double cutoff;            /* CUTOFF = 3: kernel box half-width in h         */
+  double cutoff2;           /* CUTOFF2: pair pre-filter box in h_i (8 / 12)   */
+  double list_factor;       /* 4.25: neighbour-list box in h_i (neighs.cc:84) */
+  double alpha, beta, ata;  /* alf_P, beta_P, ata_P (Monaghan viscosity)      */
+  double eps_half;          /* EPSILON_HALF (SPH-epsilon turbulence)          */
+  double prandtl;           /* PRANDTL_NUM                                     */
+  double e_smooth_ratio;    /* E_SMOOTH_RATIO                                  */
+  double heat_scale;        /* HEAT_TRANS_SCALE_RATIO                          */
+  double g;                 /* g_P (gravity along the last axis, downward)     */
+  double cfl;               /* CFL_P                                           */
+  double thresh;            /* thresh_P (adaptive-h tolerance)                 */
+  double c_smooth;          /* C_smooth_P                                      */
+  double eta;               /* eta_smooth_P                                    */
+  double ratio_sml_dx;      /* RATIO_SML_DX                                    */
+  double msfrc_thresh;      /* MSFRC_THRESH                                    */
+  double ng0, cvs, cvg, cva, rg, ra;   /* EOS constants (parameters.h:905-958) */
+  double gamma_p;           /* gamma_P: initial-energy gamma (1.4)             */
+
+  double h_air;             /* deck line 9: air smoothing length               */
+  double bucket_edge;       /* PARTICLE_DENSITY * h_air                        */
+  double dom_lo[3], dom_hi[3];   /* Ll_P, Lu_P: the simulated domain           */
+  double ext_lo[3], ext_hi[3];   /* extended domain (key normalisation, cells) */
+
+  double t_end;             /* deck line 1                                     */
+  double t_output;          /* deck line 3 (dt is clipped to hit it)           */
+  double erupt_start, erupt_end;  /* deck lines 4-5                            */
+
+  /* vent (parameters.h:944-982); the derived ones are set by psph_config_vent */
+  double vent_r, vent_v, vent_T, vent_p, vent_x, vent_y, vent_z;
+  double vent_rho, vent_e, vent_gamma, vent_c;
+  double vent_sml, vent_mass, vent_bot;
+  double reserved_d[8];
+} psph_config;
+
+/*
+ * Structure-of-arrays particle view (particle.h:34-175).  Every pointer
+ * addresses `n` elements (the caller sizes them).  key = TKey {hsfc[0],
+ * hsfc[1], add_step}.  v = state_vars[1..3], rho = state_vars[0],
+ * e = state_vars[NO_OF_EQNS-1], sv = smoothed_v, xi = mass_frac,
+ * cp = specific_heat_p, gam = gamma, c = sound_speed, T = temperature.
+ * For dim 1 only x, v[0], sv[0] are meaningful.
+ */
+typedef struct psph_soa {
+  double *x, *y, *z;
+  double *vx, *vy, *vz;
+  double *svx, *svy, *svz;
+  double *rho, *e, *m, *h, *p, *c, *T, *gam, *cp, *xi;
+  uint32_t *key0, *key1, *key2;
+  int32_t *bc, *phase, *involved;
+} psph_soa;
+
+typedef struct psph_step_out {
+  double dt;                /* step size actually taken (after clipping)       */
+  double time;              /* simulated time after the step                   */
+  int64_t step;             /* step counter after the step                     */
+  int64_t n_total;          /* particles held (all bc types)                   */
+  int64_t n_real;           /* bc in {100, 99}                                 */
+  int64_t n_injected;       /* particles added by the inlet this step          */
+  int64_t n_removed;        /* real particles that escaped the domain          */
+  int64_t n_pairs;          /* kernel-support pairs in the RHS pass            */
+} psph_step_out;
+
+typedef struct psph_ctx psph_ctx;
+
+/* ---- configuration ----------------------------------------------------- */
+/* Reference defaults for dim 3 (options.h:452-837, constant.h:109-203,
+ * parameters.h:889-1043) or dim 1 (options.h:16-450, parameters.h:1225-1446). */
+int psph_config_default(psph_config *cfg, int dim);
+/* Derived vent state: rho_v, e_v, gamma_v (parameters.h:958-975), the vent
+ * column spacing and mass (Compute_mass, sph_lib.cc:1024-1040, called from
+ * setup_erupt.cc:94-104) and the column bottom (setup_erupt.cc:93). */
+int psph_config_vent(psph_config *cfg);
+/* Extended domain and cell lattice from dom_lo/dom_hi and h_air exactly as
+ * the grid preprocessor builds it (src/preproc/preprocess.cc:88-133). */
+int psph_config_domain(psph_config *cfg);
+/* Parse a reference simulation.data deck (dataread.cc:77-245): values go
+ * through stof as in the reference.  meteo_path_out receives line 18. */
+int psph_read_deck(const char *path, psph_config *cfg, char *meteo_path_out,
+                   size_t meteo_path_cap, int32_t *meteo_rows, int32_t *meteo_cols);
+
+/* ---- inputs ------------------------------------------------------------ */
+/* Meteo table in the reference layout (meteo.cc:22-40): z[rows] in km and
+ * data[prop*rows + r] for prop = rho(kg/m3), p(hPa), T(K), q, u, v. */
+typedef struct psph_meteo {
+  int32_t rows, nprop;
+  const double *z;
+  const double *data;
+} psph_meteo;
+
+/* Read a meteo file of rows x cols numbers (readFile, sph_lib.cc:1559;
+ * Meteo::Meteo, meteo.cc:22-40).  z and data are caller buffers. */
+int psph_meteo_read(const char *path, int32_t rows, int32_t cols, double *z, double *data);
+/* Standard atmosphere at rest in that layout: T = T0 - 6.5 K/km up to 11 km,
+ * isothermal above; hydrostatic p; rho = p / (R_air T).  Rows every dz_km
+ * from 0 to z_top_km.  Returns the row count in *rows. */
+int psph_meteo_standard(double t0, double p0, double z_top_km, double dz_km,
+                        double *z, double *data, int32_t cap, int32_t *rows);
+/* Particle keys THSFC3d (hilbert.c:329-359) of positions normalised to the
+ * extended domain (add_air.cc:604-607). */
+int psph_keys(const psph_config *cfg, const double *x, const double *y, const double *z,
+              size_t n, uint32_t add_step, uint32_t *k0, uint32_t *k1, uint32_t *k2);
+/* Synthetic plume input (add_air.cc:490-813 lattice, initial_air
+ * sph_lib.cc:831-872, setup_erupt.cc:43-386 vent column): air lattice of
+ * spacing h_air over the extended domain -- real air inside dom, pressure
+ * ghosts (bc 1) in the shell above ground, wall ghosts (bc 2) below --
+ * plus the vent's erupt-ghost column.  Seeded jitter (fraction of h_air)
+ * from Philox4x32-10 keyed by lattice index.  Canonical order.
+ * With out == NULL only *n_out (an upper bound) is returned. */
+int psph_gen_plume(const psph_config *cfg, const psph_meteo *met, uint32_t seed,
+                   double jitter, psph_soa *out, size_t cap, size_t *n_out);
+/* 1D shock tube (add_air.cc:36-485): left/right state {rho, p, u}. */
+int psph_gen_shock_tube(const psph_config *cfg, const double left[3], const double right[3],
+                        double mid, int32_t n_nosave_l, int32_t n_nosave_r, int32_t n_ghost,
+                        int32_t equal_mass, psph_soa *out, size_t cap, size_t *n_out);
+
+/* ---- context ----------------------------------------------------------- */
+int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **out);
+int psph_destroy(psph_ctx *ctx);
+/* Upload n particles (any order) and reset time to t0 / step counter. */
+int psph_load(psph_ctx *ctx, const psph_soa *host, size_t n, double t0, int64_t step0);
+/* Download all particles in the reference's THashTable iteration order
+ * (thashtab.cc:209-262): ascending slot, then (key0,key1,key2). */
+int psph_read(psph_ctx *ctx, psph_soa *host, size_t cap, size_t *n_out);
+int psph_count(psph_ctx *ctx, size_t *n_total, size_t *n_real);
+int psph_time(psph_ctx *ctx, double *time, int64_t *step);
+
+/* ---- stages (reference granularity) ------------------------------------ */
+/* dt = CFL * min h/c over need_neigh particles (timestep.cc:94-135, 1D
+ * :139-183), then TimeProps::incrtime clipping (properties.h:71-82).  Only
+ * computes; psph_advance_time commits. */
+int psph_timestep(psph_ctx *ctx, double *dt_out);
+int psph_advance_time(psph_ctx *ctx, double *dt_inout);
+/* search_neighs_consth (neighs.cc:37-211): cell-key radix sort + cell
+ * ranges.  Lists are not materialised (see psph_get_neighbors). */
+int psph_neighbors(psph_ctx *ctx);
+/* adaptive_sml (neighs.cc:479-693). */
+int psph_adapt_h(psph_ctx *ctx);
+/* mom_engr_update, USE_GSPH==0 (mom_engr_update.cc:30-439) + EOS. */
+int psph_rhs(psph_ctx *ctx, double dt);
+/* smooth_density (density_massfrac.cc:25-359) + EOS. */
+int psph_density(psph_ctx *ctx);
+/* smooth_velocity (smooth_velocity.cc:25-165). */
+int psph_vsmooth(psph_ctx *ctx);
+/* update_pos (update_pos.cc:34-168) + escape removal
+ * (update_particles_in_bucket, update_pos.cc:171-463). */
+int psph_update_pos(psph_ctx *ctx, double dt);
+/* add_new_erupt (setup_erupt.cc:389-670); *added receives the count. */
+int psph_inject(psph_ctx *ctx, double dt, int64_t *added);
+/* One whole step in the order of main.cc:217-486 (3D) / 676-751 (1D). */
+int psph_step(psph_ctx *ctx, psph_step_out *out);
+int psph_run(psph_ctx *ctx, int64_t nsteps, psph_step_out *last);
+
+/* ---- parity and diagnostics -------------------------------------------- */
+/* Neighbour lists of the need_neigh particles (neighs.cc:37-211) as CSR in
+ * read order: offsets[n+1], idx[] (ascending read-order indices).  Pass
+ * idx=NULL to get only offsets (offsets[n] = total). */
+int psph_get_neighbors(psph_ctx *ctx, int64_t *offsets, size_t n_cap,
+                       uint32_t *idx, size_t idx_cap);
+/* Totals over real particles: out[0]=sum m, out[1]=sum m e,
+ * out[2]=sum m |v|^2/2, out[3..5]=sum m v, out[6]=sum m xi. FP64. */
+int psph_conservation(psph_ctx *ctx, double out[7]);
+/* Plume diagnostic: mass-weighted mean height of erupted material over real
+ * particles, H = sum m xi z / sum m xi.  No reference counterpart. */
+int psph_diag_height(psph_ctx *ctx, double *H);
+
+const char *psph_last_error(void);
+const char *psph_version(void);
+/* sizeof(psph_config), sizeof(psph_soa), sizeof(psph_step_out) for FFI
+ * bindings to check their struct layout: which = 0, 1, 2. */
+size_t psph_abi_sizeof(int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSPH_H */

@@ -1,0 +1,1523 @@
+// psph_device.hip -- the per-step SPH particle path on one MI355X (gfx950).
+//
+// Data layout: structure-of-arrays in HBM, one array per particle field, all
+// in "cell order" (particles sorted by the linear index of a uniform cell
+// grid whose edge covers the largest kernel support).  Every step:
+//   1. cell keys -> radix sort (key, index) -> gather every field (neighbours
+//      are then contiguous: cell rows along x are contiguous index ranges);
+//   2. targets (bc 100/99) are compacted and cut into "chunks" of <= 64
+//      consecutive targets inside one (y,z) cell row;
+//   3. pair passes: one 64-lane wavefront per chunk.  The chunk's candidate
+//      range for each neighbouring (y,z) row is one contiguous slice; it is
+//      staged 64 candidates at a time into LDS with coalesced loads (derived
+//      per-candidate terms such as 1/rho and p/rho^2 are computed once, by the
+//      staging lane), then every lane walks the tile with broadcast LDS reads.
+//      No list is materialised; the reference's list membership rule is
+//      evaluated per pair (same bucket, or |dx| <= 4.25 h_i per axis).
+// No MFMA: there is no dense contraction.  Reductions are deterministic
+// (min/max via integer atomics on positive doubles; sums via fixed-order
+// block partials).
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "psph_internal.h"
+
+using namespace psph;
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail(PSPH_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define PSPH_TRY(expr)            \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_) return rc_;          \
+  } while (0)
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr uint32_t kDead = 0xffffffffu;
+
+// ---------------------------------------------------------------------------
+// device-side error word: [0] = PSPH_E* code (0 = none), [1] = particle index
+// ---------------------------------------------------------------------------
+__device__ inline void raise_err(int *err, int code, uint32_t idx) {
+  if (atomicCAS(err, 0, code) == 0) err[1] = (int)idx;
+}
+
+__device__ inline bool need_neigh(int bc) { return bc == 100 || bc == 99; }
+__device__ inline bool is_real(int bc) { return bc == 100 || bc == 99; }
+__device__ inline bool contr_dens(int pg, int bc) {
+  return bc == 100 || bc == 99 || (pg >= 1 && bc == 1) || (pg >= 2 && bc == 0);
+}
+__device__ inline bool contr_vel(int bc) { return bc == 100 || bc == 99 || bc == 2; }
+
+// |dx_k / h| > cut decided exactly as the reference does (s = dx/h with a
+// correctly rounded division, then compared): far from the cut-off the
+// answer follows from |dx| vs cut*h with a 1e-12 margin; inside the margin
+// the division is performed.
+__device__ inline bool outside_cut(double dx, double h, double cut) {
+  double a = fabs(dx), b = cut * h;
+  if (a > b * (1.0 + 1e-12)) return true;
+  if (a < b * (1.0 - 1e-12)) return false;
+  return fabs(dx / h) > cut;
+}
+
+// Closed-form EOS (particle.cc:528-604).
+struct EosConst { double ng0, cvs, cvg, cva, rg, ra; int lans; };
+__device__ inline void eos_dev(const EosConst &k, double rho, double e, double xi, double &p,
+                               double &c, double &T, double &gam, double *cp) {
+  double na = 1 - xi, ng = k.ng0 * xi, ns = xi - ng;
+  double Cvm = ns * k.cvs + ng * k.cvg + na * k.cva;
+  double Rm = ng * k.rg + na * k.ra;
+  double Rmg = Rm / (ng + na);
+  gam = 1 + Rm / Cvm;
+  double desmg = rho * (na + ng);
+  p = Rmg * desmg * e / Cvm;
+  c = sqrt(gam * p / desmg);
+  T = e / Cvm;
+  if (k.lans) *cp = Cvm;
+}
+
+// ---------------------------------------------------------------------------
+// simple kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, int v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__device__ inline double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// dt candidate min h/c over need_neigh particles (timestep.cc:94-135).
+__global__ void k_dt_min(uint32_t n, const int32_t *bc, const double *h, const double *c,
+                         unsigned long long *out, int *err) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 1.0e300;
+  if (i < n && need_neigh(bc[i])) {
+    double ci = c[i];
+    if (!(ci > 0)) raise_err(err, PSPH_EINVARIANT, i);
+    else v = h[i] / ci;
+  }
+  v = wave_min(v);
+  if ((threadIdx.x & 63) == 0 && v < 1.0e300) atomicMin(out, (unsigned long long)__double_as_longlong(v));
+}
+
+// max h over live particles (the cell edge must cover every pair support)
+__global__ void k_h_max(uint32_t n, const int32_t *bc, const double *h, unsigned long long *out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (i < n && bc[i] >= 0) v = h[i];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+struct GridDesc {
+  double lo[3];
+  double inv_e;
+  int nx, ny, nz;
+  uint32_t ncells;
+};
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ void k_cellkey(uint32_t n, const double *x, const double *y, const double *z,
+                          const int32_t *bc, GridDesc g, uint32_t *key, uint32_t *idx) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  idx[i] = i;
+  if (bc[i] < 0) { key[i] = g.ncells; return; }   // removed: sorts past every cell
+  int cx = clampi((int)floor((x[i] - g.lo[0]) * g.inv_e), 0, g.nx - 1);
+  int cy = clampi((int)floor((y[i] - g.lo[1]) * g.inv_e), 0, g.ny - 1);
+  int cz = clampi((int)floor((z[i] - g.lo[2]) * g.inv_e), 0, g.nz - 1);
+  key[i] = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+}
+
+template <class T>
+__global__ void k_gather(uint32_t n, const uint32_t *perm, const T *__restrict__ src, T *__restrict__ dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+// cell_start[c] = first sorted index whose key >= c, for c in [0, ncells]
+__global__ void k_cell_start(uint32_t n, const uint32_t *key, uint32_t ncells, uint32_t *cs) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  uint32_t prev = (i == 0) ? 0u : min(key[i - 1], ncells) + 1u;
+  uint32_t cur = (i == n) ? ncells + 1u : min(key[i], ncells) + 1u;
+  for (uint32_t c = prev; c < cur; c++) cs[c] = i;
+}
+
+// reference bucket id (preprocessor cells of edge 8 h_air, preprocess.cc:88-133)
+// and the smoothing length at list time
+__global__ void k_refb(uint32_t n, const double *x, const double *y, const double *z, const double *h,
+                       double lx, double ly, double lz, double del, int bx, int by, int bz,
+                       uint32_t *refb, double *h_list) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double fx = floor((x[i] - lx) / del), fy = floor((y[i] - ly) / del), fz = floor((z[i] - lz) / del);
+  uint32_t r = kDead;
+  if (fx >= 0 && fx < bx && fy >= 0 && fy < by && fz >= 0 && fz < bz)
+    r = ((uint32_t)fz * by + (uint32_t)fy) * bx + (uint32_t)fx;
+  refb[i] = r;
+  h_list[i] = h[i];
+}
+
+__global__ void k_target_flags(uint32_t n, const int32_t *bc, uint32_t *flags) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= n) flags[i] = (i < n && need_neigh(bc[i])) ? 1u : 0u;
+}
+
+__global__ void k_targets(uint32_t n, const uint32_t *flags, const uint32_t *tpos, uint32_t *targets) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) targets[tpos[i]] = i;
+}
+
+__global__ void k_row_chunks(uint32_t nrows, int nx, const uint32_t *cs, const uint32_t *tpos,
+                             uint32_t *row_chunks) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > nrows) return;
+  if (r == nrows) { row_chunks[r] = 0; return; }
+  uint32_t a = tpos[cs[(size_t)r * nx]], b = tpos[cs[(size_t)(r + 1) * nx]];
+  row_chunks[r] = (b - a + kWave - 1) / kWave;
+}
+
+__global__ void k_chunks(uint32_t nrows, int nx, const uint32_t *cs, const uint32_t *tpos,
+                         const uint32_t *row_off, uint32_t *cfirst, uint32_t *ccount,
+                         uint32_t *crow, uint32_t *nchunks) {
+  uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) {
+    if (r == nrows) *nchunks = row_off[nrows];
+    return;
+  }
+  uint32_t a = tpos[cs[(size_t)r * nx]], b = tpos[cs[(size_t)(r + 1) * nx]];
+  uint32_t o = row_off[r];
+  for (uint32_t t = a; t < b; t += kWave) {
+    cfirst[o] = t;
+    ccount[o] = min((uint32_t)kWave, b - t);
+    crow[o] = r;
+    o++;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pair-pass skeleton
+// ---------------------------------------------------------------------------
+struct PairCommon {
+  const double *x, *y, *z;
+  const uint32_t *refb;
+  const double *h_list;
+  const uint32_t *cellkey;    // sorted cell keys
+  const uint32_t *cs;         // cell_start[ncells+1]
+  const uint32_t *targets;
+  const uint32_t *cfirst, *ccount, *crow, *nchunks;
+  int nx, ny, nz, r;
+  int list_all;
+  double list_factor;
+  int *err;
+};
+
+// Each pass supplies: Tile (LDS SoA), Tgt (registers), stage(), pair(), finish().
+template <int D, class P>
+__global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args a) {
+  __shared__ double sx[kWave], sy[kWave], sz[kWave];
+  __shared__ uint32_t srefb[kWave];
+  __shared__ typename P::Tile tile;
+  const uint32_t chunk = blockIdx.x;
+  if (chunk >= *pc.nchunks) return;
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk], row = pc.crow[chunk];
+  const int lane = threadIdx.x;
+  const bool active = (uint32_t)lane < cc;
+  const uint32_t i = pc.targets[cf + (active ? lane : 0)];
+  const int cx0 = (int)(pc.cellkey[pc.targets[cf]] % (uint32_t)pc.nx);
+  const int cx1 = (int)(pc.cellkey[pc.targets[cf + cc - 1]] % (uint32_t)pc.nx);
+  const int cy = (int)(row % (uint32_t)pc.ny), cz = (int)(row / (uint32_t)pc.ny);
+
+  double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  const uint32_t rb_i = pc.refb[i];
+  const double lf_i = pc.list_factor * pc.h_list[i];
+  typename P::Tgt t;
+  P::load_target(a, i, t);
+
+  const int xlo = max(cx0 - pc.r, 0), xhi = min(cx1 + pc.r, pc.nx - 1);
+  for (int pass = 0; pass < P::npasses(a); pass++) {
+    P::begin_pass(a, t, pass);
+    for (int dz = -pc.r; dz <= pc.r; dz++) {
+      const int zz = cz + dz;
+      if (zz < 0 || zz >= pc.nz) continue;
+      for (int dy = -pc.r; dy <= pc.r; dy++) {
+        const int yy = cy + dy;
+        if (yy < 0 || yy >= pc.ny) continue;
+        const size_t rbase = ((size_t)zz * pc.ny + yy) * pc.nx;
+        const uint32_t lo = pc.cs[rbase + xlo], hi = pc.cs[rbase + xhi + 1];
+        for (uint32_t base = lo; base < hi; base += kWave) {
+          const uint32_t j = base + lane;
+          if (j < hi) {
+            sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+            srefb[lane] = pc.refb[j];
+            P::stage(a, j, tile, lane);
+          }
+          __syncthreads();
+          const uint32_t cnt = min((uint32_t)kWave, hi - base);
+          if (active && P::want(t)) {
+            // phase 1: exact FP64 predicate -> per-lane hit mask
+            uint64_t mask = 0;
+            for (uint32_t q = 0; q < cnt; q++) {
+              double dx[3];
+              dx[0] = xi[0] - sx[q];
+              if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
+              if (P::quick_reject(a, t, tile, q, dx)) continue;
+              // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
+              bool in = pc.list_all || srefb[q] == rb_i;
+              if (!in) {
+                in = fabs(dx[0]) <= lf_i;
+                if (D == 3) in = in && fabs(dx[1]) <= lf_i && fabs(dx[2]) <= lf_i;
+              }
+              if (in && P::accept(a, t, tile, q, base + q, i, dx)) mask |= 1ull << q;
+            }
+            // phase 2: each lane walks its own hits
+            while (mask) {
+              const uint32_t q = (uint32_t)__builtin_ctzll(mask);
+              mask &= mask - 1;
+              double dx[3];
+              dx[0] = xi[0] - sx[q];
+              if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
+              P::pair(a, t, tile, q, base + q, i, dx);
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+    P::end_pass(a, t, pass, i, active);
+  }
+  if (active) P::finish(a, t, i, pc.err);
+}
+
+__device__ inline bool in_support(const double *dx, double h, int D) {
+  for (int k = 0; k < D; k++)
+    if (fabs(dx[k]) > h) return false;
+  return true;
+}
+
+constexpr double kInvSqrtPi = 0.564189583547756;   // sph_lib.cc:207
+
+// |dx_k| > 3 h (1 + 1e-12) on any axis implies fl(dx_k / h) > 3: the pair is
+// outside the kernel box and contributes exactly zero.
+template <int D>
+__device__ inline bool beyond(const double *dx, double bound) {
+  bool out = fabs(dx[0]) > bound;
+  if (D == 3) out = out || fabs(dx[1]) > bound || fabs(dx[2]) > bound;
+  return out;
+}
+constexpr double kCutMargin = 1.0 + 1e-12;
+
+// ---------------------------------------------------------------------------
+// RHS pass: mom_engr_update, USE_GSPH==0 (mom_engr_update.cc:30-439)
+// ---------------------------------------------------------------------------
+struct RhsPass {
+  struct Args {
+    const double *h, *m, *rho, *p, *c, *T, *cp, *vx, *vy, *vz, *svx, *svy, *svz, *e, *xi;
+    const int32_t *bc;
+    double *o_vx, *o_vy, *o_vz, *o_e, *o_p, *o_c, *o_T, *o_gam, *o_cp;
+    double dt, g, cutoff, cutoff2, alpha, beta, ata, eps_half, prandtl, heat_scale;
+    int lans, dim;
+    EosConst eos;
+    unsigned long long *pairs;
+  };
+  struct Tile {
+    double h[kWave], m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
+    double vx[kWave], vy[kWave], vz[kWave];
+  };
+  struct Tgt {
+    double hi, supp, rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3], rv[3], re;
+    unsigned npairs;
+    bool bad;
+  };
+  __device__ static int npasses(const Args &) { return 1; }
+  __device__ static void begin_pass(const Args &, Tgt &, int) {}
+  __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
+  __device__ static bool want(const Tgt &) { return true; }
+  __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
+    t.hi = a.h[i];
+    t.supp = a.cutoff2 * t.hi;
+    t.rhoi = a.rho[i];
+    t.bad = !(t.rhoi > 0);
+    t.Vi = 1.0 / t.rhoi;
+    t.pvsqi = a.p[i] * t.Vi * t.Vi;
+    t.ci = a.c[i]; t.Ti = a.T[i]; t.cpi = a.cp[i];
+    if (a.lans) { t.vel[0] = a.svx[i]; t.vel[1] = a.svy[i]; t.vel[2] = a.svz[i]; }
+    else { t.vel[0] = a.vx[i]; t.vel[1] = a.vy[i]; t.vel[2] = a.vz[i]; }
+    t.rv[0] = t.rv[1] = t.rv[2] = 0; t.re = 0; t.npairs = 0;
+  }
+  __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
+    double rj = a.rho[j];
+    double V = 1.0 / rj;
+    s.h[l] = a.h[j]; s.m[l] = a.m[j]; s.V[l] = V; s.PV2[l] = a.p[j] * V * V; s.rho[l] = rj;
+    s.c[l] = a.c[j]; s.T[l] = a.T[j]; s.cp[l] = a.cp[j];
+    if (a.lans) { s.vx[l] = a.svx[j]; s.vy[l] = a.svy[j]; s.vz[l] = a.svz[j]; }
+    else { s.vx[l] = a.vx[j]; s.vy[l] = a.vy[j]; s.vz[l] = a.vz[j]; }
+  }
+  template <int D>
+  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
+    return beyond<D>(dx, 0.5 * (t.hi + s.h[q]) * a.cutoff * kCutMargin);
+  }
+  // j != i, in_support(dx, CUTOFF2 h_i), rho_j > 0 asserted, d_weight != 0
+  template <int D>
+  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t j, uint32_t i,
+                                  const double *dx) {
+    if (j == i) return false;                             // dW(0) = 0
+    if (!in_support(dx, t.supp, D)) return false;
+    if (!(s.rho[q] > 0)) { t.bad = true; return false; }
+    const double h = 0.5 * (t.hi + s.h[q]);
+    for (int k = 0; k < D; k++)
+      if (outside_cut(dx[k], h, a.cutoff)) return false;
+    return true;
+  }
+  template <int D>
+  __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                const double *dx) {
+    t.npairs++;
+    const double hj = s.h[q];
+    const double h = 0.5 * (t.hi + hj);
+    const double ih = 1.0 / h;
+    double si[3], dist_sq = 0, expt = 0, norm = 1;
+    for (int k = 0; k < D; k++) {
+      si[k] = dx[k] * ih;
+      dist_sq += dx[k] * dx[k];
+      norm *= kInvSqrtPi * ih;
+      expt += si[k] * si[k];
+    }
+    const double W = norm * exp(-expt);
+    const double Vj = s.V[q];
+    double vij[3];
+    vij[0] = t.vel[0] - s.vx[q];
+    if (D == 3) { vij[1] = t.vel[1] - s.vy[q]; vij[2] = t.vel[2] - s.vz[q]; }
+    const double rhoab = 0.5 * (t.rhoi + s.rho[q]);
+    const double cab = 0.5 * (t.ci + s.c[q]);
+    // art_vis (sph_lib.cc:402-424)
+    double vrab = 0;
+    for (int k = 0; k < D; k++) vrab += dx[k] * vij[k];
+    double vis = 0;
+    if (vrab < 0) {
+      double miu = h * vrab / (dist_sq + a.ata * h * h);
+      vis = (a.beta * miu - a.alpha * cab) * miu / rhoab;
+    }
+    double dotvv = 0;
+    for (int k = 0; k < D; k++) dotvv += vij[k] * vij[k];
+    if (a.lans) vis -= a.eps_half * dotvv * Vj;           // SPH_epsilon_mom
+    const double mpv = s.m[q] * (s.PV2[q] + t.pvsqi + vis);
+    double dw[3];
+    for (int k = 0; k < D; k++) dw[k] = (-2 * si[k] * ih) * W;
+    double heat = 0;
+    if (a.lans && D == 3) {
+      // SPH_epsilon_heat_conductivity (6-arg, sph_lib.cc:1399-1439) and
+      // compute_F (sph_lib.cc:257-281); dW of the heat term equals dW
+      // (HEAT_TRANS_SCALE_RATIO = 1).
+      double dotvr = 0, wr = 0;
+      for (int k = 0; k < D; k++) { dotvr += fabs(vij[k] * dx[k]); wr += dw[k] * dx[k]; }
+      if (dotvr != 0 && wr != 0) {
+        const double cpab = 0.5 * (s.cp[q] + t.cpi);
+        const double kab = 10.0 * a.eps_half * cpab * dist_sq * dotvv / (a.prandtl * dotvr);
+        const double kij = kab / (h * cab);
+        const double F = wr / dist_sq;
+        heat = s.m[q] * Vj * t.Vi * kij * (t.Ti - s.T[q]) * F;
+      }
+    }
+    double de = 0;
+    for (int k = 0; k < D; k++) {
+      t.rv[k] -= mpv * dw[k];
+      de += 0.5 * (mpv * dw[k]) * vij[k];
+    }
+    t.re += de + heat;
+  }
+  __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
+    if (t.bad) { raise_err(err, PSPH_EINVARIANT, i); return; }
+    const double gz = -a.g, dt = a.dt;
+    // v += dt rhs_v; gravity on the last axis; e += dt (rhs_e + v_last g)
+    // (mom_engr_update.cc:372-388)
+    double nv0 = a.vx[i] + dt * (t.rv[0]);
+    double nv1 = a.vy[i] + dt * (t.rv[1]);
+    double nv2 = a.vz[i] + dt * (t.rv[2]);
+    double ne;
+    if (a.dim == 3) {
+      nv2 += dt * gz;
+      ne = a.e[i] + dt * (t.re + t.vel[2] * gz);
+    } else {
+      nv0 += dt * gz;
+      ne = a.e[i] + dt * (t.re + t.vel[0] * gz);
+    }
+    a.o_vx[i] = nv0; a.o_vy[i] = nv1; a.o_vz[i] = nv2;
+    a.o_e[i] = ne;
+    double p, c, T, gam, cp = a.cp[i];
+    eos_dev(a.eos, a.rho[i], ne, a.xi[i], p, c, T, gam, &cp);
+    a.o_p[i] = p; a.o_c[i] = c; a.o_T[i] = T; a.o_gam[i] = gam; a.o_cp[i] = cp;
+    if (t.npairs) atomicAdd(a.pairs, (unsigned long long)t.npairs);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Density pass: smooth_density (density_massfrac.cc:25-359)
+// ---------------------------------------------------------------------------
+struct DensPass {
+  struct Args {
+    const double *h, *m, *rho, *e;
+    const int32_t *bc, *phase;
+    double *o_rho, *o_e, *xi, *p, *c, *T, *gam, *cp;
+    int32_t *inv;
+    double cutoff, cutoff2, e_ratio, msfrc_thresh;
+    int pghost, energy_smooth;
+    EosConst eos;
+  };
+  struct Tile {
+    double h[kWave], m[kWave], rho[kWave], e[kWave];
+    int ok[kWave];   // phase (1/2) if contr_dens, else 0
+  };
+  struct Tgt { double hi, supp, r1, r2, wn, en; };
+  __device__ static int npasses(const Args &) { return 1; }
+  __device__ static void begin_pass(const Args &, Tgt &, int) {}
+  __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
+  __device__ static bool want(const Tgt &) { return true; }
+  __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
+    t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi; t.r1 = t.r2 = t.wn = t.en = 0;
+  }
+  __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
+    s.h[l] = a.h[j]; s.m[l] = a.m[j]; s.rho[l] = a.rho[j]; s.e[l] = a.e[j];
+    int ph = a.phase[j];
+    s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
+  }
+  template <int D>
+  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
+    return !s.ok[q] || beyond<D>(dx, 0.5 * (t.hi + s.h[q]) * a.cutoff * kCutMargin);
+  }
+  // contr_dens and phase, in_support(ds, CUTOFF2 h_i), weight != 0 at h_ij.
+  // The energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
+  template <int D>
+  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                  const double *ds) {
+    if (!in_support(ds, t.supp, D)) return false;
+    const double hij = 0.5 * (t.hi + s.h[q]);
+    for (int k = 0; k < D; k++)
+      if (outside_cut(ds[k], hij, a.cutoff)) return false;
+    return true;
+  }
+  template <int D>
+  __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                const double *ds) {
+    const double hij = 0.5 * (t.hi + s.h[q]);
+    const double ih = 1.0 / hij;
+    const double mj = s.m[q];
+    if (a.energy_smooth) {
+      // weight(4 ds / h_ij, h_ij / 4): support |ds| <= 0.75 h_ij
+      bool out = false;
+      for (int k = 0; k < D; k++) out = out || outside_cut(a.e_ratio * ds[k], hij, a.cutoff);
+      if (!out) {
+        const double ihe = 1.0 / (hij / a.e_ratio);
+        double ne = 1, ee = 0;
+        for (int k = 0; k < D; k++) {
+          double se = a.e_ratio * ds[k] * ih;
+          ne *= kInvSqrtPi * ihe;
+          ee += se * se;
+        }
+        const double wm_e = ne * exp(-ee) * mj;
+        const double rj = s.rho[q];
+        t.wn += wm_e / rj;
+        t.en += wm_e * s.e[q] / rj;
+      }
+    }
+    double expt = 0, norm = 1;
+    for (int k = 0; k < D; k++) {
+      double sk = ds[k] * ih;
+      norm *= kInvSqrtPi * ih;
+      expt += sk * sk;
+    }
+    const double wm = norm * exp(-expt) * mj;
+    if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
+  }
+  __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
+    double density = 0.0;
+    density += t.r1;
+    density += t.r2;
+    const double xi = t.r2 / density;
+    double e = a.e[i];
+    if (a.energy_smooth) {
+      if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
+      e = t.en / t.wn;
+    }
+    if (!(density > 0) || !(xi <= 1)) { raise_err(err, PSPH_EINVARIANT, i); return; }
+    a.o_rho[i] = density;
+    a.o_e[i] = e;
+    a.xi[i] = xi;
+    if (xi >= a.msfrc_thresh && a.inv[i] != 2) a.inv[i] = 2;
+    double p, c, T, gam, cp = a.cp[i];
+    eos_dev(a.eos, density, e, xi, p, c, T, gam, &cp);
+    a.p[i] = p; a.c[i] = c; a.T[i] = T; a.gam[i] = gam; a.cp[i] = cp;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// LANS velocity filter: smooth_velocity (smooth_velocity.cc:25-165)
+// ---------------------------------------------------------------------------
+struct VsmPass {
+  struct Args {
+    const double *h, *m, *rho, *vx, *vy, *vz;
+    const int32_t *bc;
+    double *svx, *svy, *svz;
+    double cutoff, cutoff2;
+  };
+  struct Tile {
+    double mr[kWave], vx[kWave], vy[kWave], vz[kWave];
+    int ok[kWave];
+  };
+  struct Tgt { double hi, ih, bound, supp, norm, v[3], dv[3], wn; };
+  __device__ static int npasses(const Args &) { return 1; }
+  __device__ static void begin_pass(const Args &, Tgt &, int) {}
+  __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
+  __device__ static bool want(const Tgt &) { return true; }
+  __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
+    t.hi = a.h[i]; t.ih = 1.0 / t.hi; t.bound = a.cutoff * t.hi * kCutMargin; t.supp = a.cutoff2 * t.hi;
+    t.norm = kInvSqrtPi * t.ih;   // per-axis factor of the kernel norm
+    t.v[0] = a.vx[i]; t.v[1] = a.vy ? a.vy[i] : 0; t.v[2] = a.vz ? a.vz[i] : 0;
+    t.dv[0] = t.dv[1] = t.dv[2] = 0; t.wn = 0;
+  }
+  __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
+    s.mr[l] = a.m[j] / a.rho[j];
+    s.vx[l] = a.vx[j];
+    s.vy[l] = a.vy ? a.vy[j] : 0;
+    s.vz[l] = a.vz ? a.vz[j] : 0;
+    s.ok[l] = contr_vel(a.bc[j]);
+  }
+  template <int D>
+  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
+    return !s.ok[q] || beyond<D>(dx, t.bound);
+  }
+  template <int D>
+  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                  const double *ds) {
+    if (!in_support(ds, t.supp, D)) return false;
+    for (int k = 0; k < D; k++)
+      if (outside_cut(ds[k], t.hi, a.cutoff)) return false;
+    return true;
+  }
+  template <int D>
+  __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                const double *ds) {
+    double expt = 0, norm = 1;
+    for (int k = 0; k < D; k++) {
+      double sk = ds[k] * t.ih;
+      norm *= t.norm;
+      expt += sk * sk;
+    }
+    const double mvw = norm * exp(-expt) * s.mr[q];
+    t.dv[0] += mvw * (s.vx[q] - t.v[0]);
+    if (D == 3) {
+      t.dv[1] += mvw * (s.vy[q] - t.v[1]);
+      t.dv[2] += mvw * (s.vz[q] - t.v[2]);
+    }
+    t.wn += mvw;
+  }
+  __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
+    if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
+    a.svx[i] = t.v[0] + t.dv[0] / t.wn;
+    if (a.svy) { a.svy[i] = t.v[1] + t.dv[1] / t.wn; a.svz[i] = t.v[2] + t.dv[2] / t.wn; }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Adaptive smoothing length: adaptive_sml (neighs.cc:479-693).  One launch
+// per iteration of the reference's loop; h is updated in place (no pass reads
+// another particle's h) and done[] marks particles whose |dh/h| < thresh.
+// ---------------------------------------------------------------------------
+struct AdaptPass {
+  struct Args {
+    const double *m;
+    const int32_t *bc;
+    double *h;
+    uint32_t *done;
+    double cutoff, cutoff2, c_smooth, eta, ratio, thresh;
+    int pghost, dim;
+  };
+  struct Tile {
+    double m[kWave];
+    int ok[kWave];
+  };
+  struct Tgt { double mi, hcur, Hs, iHs, bound, supp, rho_star; bool done; };
+  __device__ static int npasses(const Args &) { return 1; }
+  __device__ static void begin_pass(const Args &a, Tgt &t, int) {
+    t.Hs = t.hcur * a.c_smooth;
+    t.iHs = 1.0 / t.Hs;
+    t.bound = a.cutoff * t.Hs * kCutMargin;
+    t.supp = a.cutoff2 * t.Hs;
+    t.rho_star = 0;
+  }
+  __device__ static bool want(const Tgt &t) { return !t.done; }
+  __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
+    t.mi = a.m[i]; t.hcur = a.h[i]; t.done = a.done[i] != 0;
+  }
+  __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
+    s.m[l] = a.m[j];
+    s.ok[l] = contr_dens(a.pghost, a.bc[j]);
+  }
+  template <int D>
+  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
+    return !s.ok[q] || beyond<D>(dx, t.bound);
+  }
+  template <int D>
+  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                  const double *dx) {
+    if (!in_support(dx, t.supp, D)) return false;
+    for (int k = 0; k < D; k++)
+      if (outside_cut(dx[k], t.Hs, a.cutoff)) return false;
+    return true;
+  }
+  template <int D>
+  __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
+                                const double *dx) {
+    double expt = 0, norm = 1;
+    for (int k = 0; k < D; k++) {
+      double sk = dx[k] * t.iHs;
+      norm *= kInvSqrtPi * t.iHs;
+      expt += sk * sk;
+    }
+    t.rho_star += norm * exp(-expt) * s.m[q];
+  }
+  __device__ static void end_pass(const Args &a, Tgt &t, int, uint32_t, bool active) {
+    if (!active || t.done) return;
+    double Hold = t.hcur;
+    double Hnew = (a.dim == 3) ? a.eta * cbrt(t.mi / t.rho_star) : a.eta * (t.mi / t.rho_star);
+    t.hcur = a.ratio * Hnew;
+    if (fabs((Hnew - Hold) / Hold) < a.thresh) t.done = true;
+  }
+  __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *) {
+    a.h[i] = t.hcur;
+    a.done[i] = t.done ? 1u : 0u;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// List materialisation (parity checks only): the candidates the skeleton's
+// membership rule accepts are exactly the reference's list (neighs.cc:37-211).
+// mode 0 counts, mode 1 writes device indices at offsets[i].
+// ---------------------------------------------------------------------------
+struct ListPass {
+  struct Args {
+    int mode;
+    uint32_t *count;
+    const uint32_t *offset;
+    uint32_t *out;
+  };
+  struct Tile { int dummy; };
+  struct Tgt { uint32_t n, o; };
+  __device__ static int npasses(const Args &) { return 1; }
+  __device__ static void begin_pass(const Args &, Tgt &, int) {}
+  __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
+  __device__ static bool want(const Tgt &) { return true; }
+  __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
+    t.n = 0;
+    t.o = a.mode ? a.offset[i] : 0;
+  }
+  __device__ static void stage(const Args &, uint32_t, Tile &, int) {}
+  template <int D>
+  __device__ static bool reject_d(const Args &, const Tgt &, const Tile &, uint32_t, const double *) {
+    return false;
+  }
+  template <int D>
+  __device__ static bool accept_d(const Args &, Tgt &, const Tile &, uint32_t, uint32_t, uint32_t,
+                                  const double *) {
+    return true;
+  }
+  template <int D>
+  __device__ static void pair_d(const Args &a, Tgt &t, const Tile &, uint32_t, uint32_t j, uint32_t,
+                                const double *) {
+    if (a.mode) a.out[t.o + t.n] = j;
+    t.n++;
+  }
+  __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *) {
+    if (!a.mode) a.count[i] = t.n;
+  }
+};
+
+// Wrap a pass so the skeleton can call P::pair with the dimension bound.
+template <int D, class P>
+struct Bound : P {
+  __device__ static bool quick_reject(const typename P::Args &a, const typename P::Tgt &t,
+                                      const typename P::Tile &s, uint32_t q, const double *dx) {
+    return P::template reject_d<D>(a, t, s, q, dx);
+  }
+  __device__ static bool accept(const typename P::Args &a, typename P::Tgt &t, const typename P::Tile &s,
+                                uint32_t q, uint32_t j, uint32_t i, const double *dx) {
+    return P::template accept_d<D>(a, t, s, q, j, i, dx);
+  }
+  __device__ static void pair(const typename P::Args &a, typename P::Tgt &t, const typename P::Tile &s,
+                             uint32_t q, uint32_t j, uint32_t i, const double *dx) {
+    P::template pair_d<D>(a, t, s, q, j, i, dx);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Position update + escape (update_pos.cc:34-168, bucket.cc:435-452)
+// ---------------------------------------------------------------------------
+__global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double dt, double *x,
+                             double *y, double *z, const double *vx, const double *vy,
+                             const double *vz, const double *svx, const double *svy,
+                             const double *svz, int32_t *bc, double *h, int32_t *inv,
+                             double ground, double h_new, double lo0, double lo1, double lo2,
+                             double hi0, double hi1, double hi2, unsigned long long *dead) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b = bc[i];
+  if (b == 100 || b == 99) {
+    if (dim == 3) {
+      double ux = lans ? svx[i] : vx[i], uy = lans ? svy[i] : vy[i], uz = lans ? svz[i] : vz[i];
+      x[i] = x[i] + dt * ux; y[i] = y[i] + dt * uy; z[i] = z[i] + dt * uz;
+    } else {
+      x[i] = x[i] + dt * (lans ? svx[i] : vx[i]);
+    }
+  } else if (b == 0 && erupting) {
+    x[i] = x[i] + dt * vx[i];
+    if (dim == 3) {
+      y[i] = y[i] + dt * vy[i];
+      z[i] = z[i] + dt * vz[i];
+      if (z[i] >= ground) { b = 100; bc[i] = 100; h[i] = h_new; inv[i] = 2; }
+    }
+  }
+  if (dim == 3 && (b == 100 || b == 99)) {
+    const double TINY = 1.0E-08;
+    bool esc = (x[i] - lo0) < -TINY || (y[i] - lo1) < -TINY || (z[i] - lo2) < -TINY ||
+               (x[i] - hi0) > TINY || (y[i] - hi1) > TINY || (z[i] - hi2) > TINY;
+    if (esc) { bc[i] = -1; atomicAdd(dead, 1ull); }
+  }
+}
+
+// fixed-order block partial sums (deterministic)
+constexpr int kRedBlocks = 512;
+__global__ void k_conservation(uint32_t n, const int32_t *bc, const double *m, const double *e,
+                               const double *vx, const double *vy, const double *vz,
+                               const double *xi, const double *z, double *part) {
+  __shared__ double sh[8][256];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int b = bc[i];
+    if (!(b == 100 || b == 99)) continue;
+    double mi = m[i], v0 = vx[i], v1 = vy ? vy[i] : 0, v2 = vz ? vz[i] : 0;
+    acc[0] += mi;
+    acc[1] += mi * e[i];
+    acc[2] += 0.5 * mi * (v0 * v0 + v1 * v1 + v2 * v2);
+    acc[3] += mi * v0; acc[4] += mi * v1; acc[5] += mi * v2;
+    acc[6] += mi * xi[i];
+    acc[7] += mi * xi[i] * (z ? z[i] : 0);
+  }
+  for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 8; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; k++) part[blockIdx.x * 8 + k] = sh[k][0];
+}
+
+inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ===========================================================================
+// Context
+// ===========================================================================
+#define PSPH_DFIELDS(F) F(x) F(y) F(z) F(vx) F(vy) F(vz) F(svx) F(svy) F(svz) F(rho) F(e) F(m) \
+  F(h) F(p) F(c) F(T) F(gam) F(cp) F(xi)
+#define PSPH_UFIELDS(F) F(k0) F(k1) F(k2)
+#define PSPH_IFIELDS(F) F(bc) F(phase) F(inv)
+
+struct DevFields {
+#define D_(a) double *a = nullptr;
+#define U_(a) uint32_t *a = nullptr;
+#define I_(a) int32_t *a = nullptr;
+  PSPH_DFIELDS(D_) PSPH_UFIELDS(U_) PSPH_IFIELDS(I_)
+#undef D_
+#undef U_
+#undef I_
+};
+
+struct psph_ctx {
+  psph_config cfg{};
+  hipStream_t stream = nullptr;
+  size_t cap = 0, n = 0;
+  DevFields f;
+  double *dsp[10] = {};        // spare double arrays (gather / double buffers)
+  uint32_t *usp = nullptr;     // spare u32 array
+  int32_t *isp = nullptr;      // spare i32 array
+  double *h_list = nullptr;
+  uint32_t *refb = nullptr, *key_a = nullptr, *key_b = nullptr, *idx_a = nullptr, *idx_b = nullptr;
+  uint32_t *flags = nullptr, *tpos = nullptr, *targets = nullptr;
+  uint32_t *cfirst = nullptr, *ccount = nullptr, *crow = nullptr;
+  uint32_t *cs = nullptr, *row_chunks = nullptr, *row_off = nullptr, *nchunks = nullptr;
+  size_t cs_cap = 0, rows_cap = 0, chunk_cap = 0;
+  void *cub = nullptr;
+  size_t cub_bytes = 0;
+  unsigned long long *d_u64 = nullptr;   // [0] dt bits [1] hmax bits [2] pairs [3] dead
+  int *d_err = nullptr;
+  double *d_part = nullptr;
+  // grid of the last neighbour search
+  bool grid_valid = false;
+  GridDesc grid{};
+  int r_main = 1;
+  double h_max = 0;
+  // clock (TimeProps)
+  double time = 0, next_output = 0;
+  int64_t step = 0;
+  int ioutput = 0;
+  int64_t last_injected = 0, last_removed = 0;
+  unsigned long long last_pairs = 0;
+  HostParticles inj;
+};
+
+static int dmalloc(void **p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess) return fail(PSPH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return 0;
+}
+
+static int check_err(psph_ctx *c, const char *stage) {
+  int he[2];
+  HIP_TRY(hipMemcpyAsync(he, c->d_err, sizeof he, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (he[0]) {
+    int code = he[0];
+    hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, c->stream, c->d_err, 0, 2);
+    return fail(code, "%s: invariant violated at device particle %d (density<=0, mass fraction>1, "
+                      "wnorm<=0 or sound speed<=0)", stage, he[1]);
+  }
+  return 0;
+}
+
+static int ensure_cub(psph_ctx *c, size_t bytes) {
+  if (bytes <= c->cub_bytes) return 0;
+  if (c->cub) hipFree(c->cub);
+  c->cub = nullptr;
+  PSPH_TRY(dmalloc(&c->cub, bytes));
+  c->cub_bytes = bytes;
+  return 0;
+}
+
+extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **out) {
+  if (!cfg || !out) return fail(PSPH_EINVAL, "null argument");
+  if (cfg->dim != 1 && cfg->dim != 3) return fail(PSPH_EINVAL, "dim must be 1 or 3");
+  if (cfg->precision != PSPH_FP64) return fail(PSPH_ENOTSUP, "only the FP64 form is built in this library version");
+  if (cfg->kernel != PSPH_KERNEL_GAUSS_BOX) return fail(PSPH_ENOTSUP, "only the Gaussian box kernel is built");
+  if (!(cfg->h_air > 0)) return fail(PSPH_EINVAL, "h_air must be > 0");
+  if (cfg->dim == 3 && !(cfg->bucket_edge > 0)) return fail(PSPH_EINVAL, "call psph_config_domain first");
+  if (capacity < 64) capacity = 64;
+  if (capacity > 0xffffff00u) return fail(PSPH_EINVAL, "capacity above 2^32 particles");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(PSPH_EHIP, "no HIP device");
+  psph_ctx *c = new psph_ctx();
+  c->cfg = *cfg;
+  c->cap = capacity;
+  int rc = 0;
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return fail(PSPH_EHIP, "hipStreamCreate"); }
+#define A_(a) if (!rc) rc = dmalloc((void **)&c->f.a, capacity * sizeof(*c->f.a));
+  PSPH_DFIELDS(A_) PSPH_UFIELDS(A_) PSPH_IFIELDS(A_)
+#undef A_
+  for (int k = 0; k < 10 && !rc; k++) rc = dmalloc((void **)&c->dsp[k], capacity * 8);
+  if (!rc) rc = dmalloc((void **)&c->usp, capacity * 4);
+  if (!rc) rc = dmalloc((void **)&c->isp, capacity * 4);
+  if (!rc) rc = dmalloc((void **)&c->h_list, capacity * 8);
+  uint32_t **u32s[] = {&c->refb, &c->key_a, &c->key_b, &c->idx_a, &c->idx_b, &c->targets};
+  for (auto pp : u32s)
+    if (!rc) rc = dmalloc((void **)pp, capacity * 4);
+  if (!rc) rc = dmalloc((void **)&c->flags, (capacity + 1) * 4);
+  if (!rc) rc = dmalloc((void **)&c->tpos, (capacity + 1) * 4);
+  if (!rc) rc = dmalloc((void **)&c->nchunks, 4);
+  if (!rc) rc = dmalloc((void **)&c->d_u64, 8 * 8);
+  if (!rc) rc = dmalloc((void **)&c->d_err, 2 * sizeof(int));
+  if (!rc) rc = dmalloc((void **)&c->d_part, kRedBlocks * 8 * 8);
+  if (rc) { psph_destroy(c); return rc; }
+  hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, c->stream, c->d_err, 0, 2);
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64, 0ull, 8);
+  c->next_output = cfg->t_output;
+  *out = c;
+  return 0;
+}
+
+extern "C" int psph_destroy(psph_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+#define F_(a) if (c->f.a) hipFree(c->f.a);
+  PSPH_DFIELDS(F_) PSPH_UFIELDS(F_) PSPH_IFIELDS(F_)
+#undef F_
+  for (auto p : c->dsp) if (p) hipFree(p);
+  void *ptrs[] = {c->usp, c->isp, c->h_list, c->refb, c->key_a, c->key_b, c->idx_a, c->idx_b,
+                  c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->crow, c->cs,
+                  c->row_chunks, c->row_off, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part};
+  for (auto p : ptrs) if (p) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+static int upload(psph_ctx *c, const HostParticles &P, size_t at) {
+  size_t n = P.size();
+  if (!n) return 0;
+#define U_(a) HIP_TRY(hipMemcpyAsync(c->f.a + at, P.a.data(), n * sizeof(P.a[0]), hipMemcpyHostToDevice, c->stream));
+  PSPH_DFIELDS(U_) PSPH_UFIELDS(U_)
+  U_(bc) U_(phase)
+#undef U_
+  HIP_TRY(hipMemcpyAsync(c->f.inv + at, P.inv.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int psph_load(psph_ctx *c, const psph_soa *host, size_t n, double t0, int64_t step0) {
+  if (!c || !host) return fail(PSPH_EINVAL, "null argument");
+  if (n > c->cap) return fail(PSPH_ECAPACITY, "load: %zu particles exceed capacity %zu", n, c->cap);
+  if (!host->x || !host->rho || !host->m || !host->h || !host->bc) return fail(PSPH_EINVAL, "load: x, rho, m, h and bc are required");
+  HostParticles P;
+  P.from_soa(host, n);
+  PSPH_TRY(upload(c, P, 0));
+  c->n = n;
+  c->time = t0;
+  c->step = step0;
+  c->ioutput = 0;
+  c->next_output = c->cfg.t_output;
+  while (c->next_output <= t0 && c->cfg.t_output > 0) { c->ioutput++; c->next_output = (c->ioutput + 1) * c->cfg.t_output; }
+  c->grid_valid = false;
+  return 0;
+}
+
+extern "C" int psph_count(psph_ctx *c, size_t *n_total, size_t *n_real) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  std::vector<int32_t> bc(c->n);
+  if (c->n) HIP_TRY(hipMemcpy(bc.data(), c->f.bc, c->n * 4, hipMemcpyDeviceToHost));
+  size_t nt = 0, nr = 0;
+  for (auto b : bc) { if (b >= 0) nt++; if (b == 100 || b == 99) nr++; }
+  if (n_total) *n_total = nt;
+  if (n_real) *n_real = nr;
+  return 0;
+}
+
+extern "C" int psph_time(psph_ctx *c, double *time, int64_t *step) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (time) *time = c->time;
+  if (step) *step = c->step;
+  return 0;
+}
+
+static int download(psph_ctx *c, HostParticles &P) {
+  size_t n = c->n;
+  P.resize(n);
+  if (!n) return 0;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+#define D_(a) HIP_TRY(hipMemcpy(P.a.data(), c->f.a, n * sizeof(P.a[0]), hipMemcpyDeviceToHost));
+  PSPH_DFIELDS(D_) PSPH_UFIELDS(D_) PSPH_IFIELDS(D_)
+#undef D_
+  return 0;
+}
+
+extern "C" int psph_read(psph_ctx *c, psph_soa *host, size_t cap, size_t *n_out) {
+  if (!c || !n_out) return fail(PSPH_EINVAL, "null argument");
+  HostParticles P;
+  PSPH_TRY(download(c, P));
+  // drop removed particles, then THashTable order
+  std::vector<uint32_t> keep;
+  keep.reserve(P.size());
+  for (size_t i = 0; i < P.size(); i++) if (P.bc[i] >= 0) keep.push_back((uint32_t)i);
+  if (keep.size() != P.size()) { P.permute(keep); }
+  auto perm = canonical_order(c->cfg, P.k0.data(), P.k1.data(), P.k2.data(), P.size());
+  P.permute(perm);
+  *n_out = P.size();
+  if (!host) return 0;
+  if (P.size() > cap) return fail(PSPH_ECAPACITY, "read: %zu particles, buffer holds %zu", P.size(), cap);
+  P.to_soa(host);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// stages
+// ---------------------------------------------------------------------------
+extern "C" int psph_timestep(psph_ctx *c, double *dt_out) {
+  if (!c || !dt_out) return fail(PSPH_EINVAL, "null argument");
+  unsigned long long init = (unsigned long long)0x7FF0000000000000ull;   // +inf bits
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64, init, 1);
+  if (c->n) hipLaunchKernelGGL(k_dt_min, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->f.c, c->d_u64, c->d_err);
+  unsigned long long bits;
+  HIP_TRY(hipMemcpyAsync(&bits, c->d_u64, 8, hipMemcpyDeviceToHost, c->stream));
+  PSPH_TRY(check_err(c, "timestep"));
+  double mn;
+  std::memcpy(&mn, &bits, 8);
+  double dt = 1.0E+10;                    // timestep.cc:99
+  if (mn < dt) dt = mn;
+  *dt_out = c->cfg.cfl * dt;
+  return 0;
+}
+
+extern "C" int psph_advance_time(psph_ctx *c, double *dt) {   // TimeProps::incrtime
+  if (!c || !dt) return fail(PSPH_EINVAL, "null argument");
+  if ((c->time + (*dt)) > c->next_output) (*dt) = c->next_output - c->time;
+  if ((c->time + (*dt)) > c->cfg.t_end) (*dt) = c->cfg.t_end - c->time;
+  c->time += (*dt);
+  c->step++;
+  return 0;
+}
+
+static int build_grid(psph_ctx *c) {
+  const psph_config &k = c->cfg;
+  // h_max over live particles
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 1, 0ull, 1);
+  if (c->n) hipLaunchKernelGGL(k_h_max, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->d_u64 + 1);
+  unsigned long long bits;
+  HIP_TRY(hipMemcpyAsync(&bits, c->d_u64 + 1, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double hmax;
+  std::memcpy(&hmax, &bits, 8);
+  if (!(hmax > 0) || !std::isfinite(hmax)) hmax = k.h_air;
+  c->h_max = hmax;
+  GridDesc g{};
+  if (k.list_all || k.dim == 1) {
+    g.lo[0] = g.lo[1] = g.lo[2] = 0;
+    g.inv_e = 0;
+    g.nx = g.ny = g.nz = 1;
+    c->r_main = 0;
+  } else {
+    // cell edge covers every kernel support of the RHS / density / velocity
+    // passes: |dx_k| <= 3 (h_i + h_j) / 2 <= 3 h_max
+    double E = k.cutoff * hmax * (1 + 1e-9);
+    int n3[3];
+    for (int d = 0; d < 3; d++) {
+      g.lo[d] = k.ext_lo[d] - E;
+      n3[d] = (int)std::ceil((k.ext_hi[d] - k.ext_lo[d] + 2 * E) / E);
+      if (n3[d] < 1) n3[d] = 1;
+    }
+    double ncell = (double)n3[0] * n3[1] * n3[2];
+    if (ncell > 2.0e8) return fail(PSPH_ECAPACITY, "cell grid of %.3g cells (h_max %.4g) too large", ncell, hmax);
+    g.inv_e = 1.0 / E;
+    g.nx = n3[0]; g.ny = n3[1]; g.nz = n3[2];
+    c->r_main = 1;
+  }
+  g.ncells = (uint32_t)g.nx * g.ny * g.nz;
+  c->grid = g;
+  size_t need = (size_t)g.ncells + 1;
+  if (need > c->cs_cap) {
+    if (c->cs) hipFree(c->cs);
+    c->cs = nullptr;
+    PSPH_TRY(dmalloc((void **)&c->cs, need * 4));
+    c->cs_cap = need;
+  }
+  size_t nrows = (size_t)g.ny * g.nz;
+  if (nrows + 1 > c->rows_cap) {
+    if (c->row_chunks) hipFree(c->row_chunks);
+    if (c->row_off) hipFree(c->row_off);
+    c->row_chunks = c->row_off = nullptr;
+    PSPH_TRY(dmalloc((void **)&c->row_chunks, (nrows + 1) * 4));
+    PSPH_TRY(dmalloc((void **)&c->row_off, (nrows + 1) * 4));
+    c->rows_cap = nrows + 1;
+  }
+  size_t chunks_ub = c->n / kWave + nrows + 1;
+  if (chunks_ub > c->chunk_cap) {
+    if (c->cfirst) hipFree(c->cfirst);
+    if (c->ccount) hipFree(c->ccount);
+    if (c->crow) hipFree(c->crow);
+    c->cfirst = c->ccount = c->crow = nullptr;
+    PSPH_TRY(dmalloc((void **)&c->cfirst, chunks_ub * 4));
+    PSPH_TRY(dmalloc((void **)&c->ccount, chunks_ub * 4));
+    PSPH_TRY(dmalloc((void **)&c->crow, chunks_ub * 4));
+    c->chunk_cap = chunks_ub;
+  }
+  return 0;
+}
+
+extern "C" int psph_neighbors(psph_ctx *c) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  PSPH_TRY(build_grid(c));
+  const GridDesc &g = c->grid;
+  uint32_t n = (uint32_t)c->n;
+  hipStream_t s = c->stream;
+  if (n) {
+    hipLaunchKernelGGL(k_cellkey, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.bc, g, c->key_a, c->idx_a);
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= (uint64_t)g.ncells) end_bit++;
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->key_a, c->key_b, c->idx_a, c->idx_b, (int)n, 0, end_bit, s));
+    PSPH_TRY(ensure_cub(c, bytes));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->cub, bytes, c->key_a, c->key_b, c->idx_a, c->idx_b, (int)n, 0, end_bit, s));
+    // gather every field into cell order
+#define G_(a) { hipLaunchKernelGGL(k_gather<double>, blocks(n), 256, 0, s, n, c->idx_b, c->f.a, c->dsp[0]); std::swap(c->f.a, c->dsp[0]); }
+    PSPH_DFIELDS(G_)
+#undef G_
+#define G_(a) { hipLaunchKernelGGL(k_gather<uint32_t>, blocks(n), 256, 0, s, n, c->idx_b, c->f.a, c->usp); std::swap(c->f.a, c->usp); }
+    PSPH_UFIELDS(G_)
+#undef G_
+#define G_(a) { hipLaunchKernelGGL(k_gather<int32_t>, blocks(n), 256, 0, s, n, c->idx_b, c->f.a, c->isp); std::swap(c->f.a, c->isp); }
+    PSPH_IFIELDS(G_)
+#undef G_
+    std::swap(c->key_a, c->key_b);   // key_a: sorted keys
+  }
+  // drop removed particles (their key == ncells sorts them to the tail)
+  unsigned long long dead = 0;
+  HIP_TRY(hipMemcpyAsync(&dead, c->d_u64 + 3, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (dead) {
+    if (dead > n) return fail(PSPH_EINVARIANT, "removed-particle count %llu exceeds %u", dead, n);
+    n -= (uint32_t)dead;
+    c->n = n;
+    hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, c->d_u64 + 3, 0ull, 1);
+  }
+  hipLaunchKernelGGL(k_cell_start, blocks(n + 1), 256, 0, s, n, c->key_a, g.ncells, c->cs);
+  // reference buckets (list membership) and h at list time
+  int nb[3] = {1, 1, 1};
+  double del = c->cfg.bucket_edge > 0 ? c->cfg.bucket_edge : 1.0;
+  if (c->cfg.dim == 3)
+    for (int d = 0; d < 3; d++) nb[d] = (int)std::llround((c->cfg.ext_hi[d] - c->cfg.ext_lo[d]) / del);
+  if (n) hipLaunchKernelGGL(k_refb, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.h,
+                            c->cfg.ext_lo[0], c->cfg.ext_lo[1], c->cfg.ext_lo[2], del, nb[0], nb[1], nb[2],
+                            c->refb, c->h_list);
+  // targets and chunks
+  hipLaunchKernelGGL(k_target_flags, blocks(n + 1), 256, 0, s, n, c->f.bc, c->flags);
+  size_t bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->flags, c->tpos, (int)n + 1, s));
+  PSPH_TRY(ensure_cub(c, bytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, c->flags, c->tpos, (int)n + 1, s));
+  if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, c->flags, c->tpos, c->targets);
+  uint32_t nrows = (uint32_t)g.ny * g.nz;
+  hipLaunchKernelGGL(k_row_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, c->cs, c->tpos, c->row_chunks);
+  bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->row_chunks, c->row_off, (int)nrows + 1, s));
+  PSPH_TRY(ensure_cub(c, bytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, c->row_chunks, c->row_off, (int)nrows + 1, s));
+  hipLaunchKernelGGL(k_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, c->cs, c->tpos, c->row_off,
+                     c->cfirst, c->ccount, c->crow, c->nchunks);
+  HIP_TRY(hipGetLastError());
+  c->grid_valid = true;
+  return 0;
+}
+
+static PairCommon make_common(psph_ctx *c, int r) {
+  PairCommon pc;
+  pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z;
+  pc.refb = c->refb; pc.h_list = c->h_list;
+  pc.cellkey = c->key_a; pc.cs = c->cs; pc.targets = c->targets;
+  pc.cfirst = c->cfirst; pc.ccount = c->ccount; pc.crow = c->crow; pc.nchunks = c->nchunks;
+  pc.nx = c->grid.nx; pc.ny = c->grid.ny; pc.nz = c->grid.nz; pc.r = r;
+  pc.list_all = (c->cfg.list_all || c->cfg.dim == 1) ? 1 : 0;
+  pc.list_factor = c->cfg.list_factor;
+  pc.err = c->d_err;
+  return pc;
+}
+
+static unsigned pair_grid(psph_ctx *c) {
+  return (unsigned)(c->n / kWave + (size_t)c->grid.ny * c->grid.nz + 1);
+}
+
+template <class P>
+static void launch_pairs(psph_ctx *c, const PairCommon &pc, const typename P::Args &a) {
+  unsigned nb = pair_grid(c);
+  if (c->cfg.dim == 3)
+    hipLaunchKernelGGL((k_pairs<3, Bound<3, P>>), nb, kWave, 0, c->stream, pc, a);
+  else
+    hipLaunchKernelGGL((k_pairs<1, Bound<1, P>>), nb, kWave, 0, c->stream, pc, a);
+}
+
+static EosConst eos_const(const psph_config &k) {
+  return EosConst{k.ng0, k.cvs, k.cvg, k.cva, k.rg, k.ra, k.lans};
+}
+
+static int reduce_hmax(psph_ctx *c, double *out) {
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 1, 0ull, 1);
+  if (c->n) hipLaunchKernelGGL(k_h_max, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->d_u64 + 1);
+  unsigned long long bits;
+  HIP_TRY(hipMemcpyAsync(&bits, c->d_u64 + 1, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::memcpy(out, &bits, 8);
+  return 0;
+}
+
+extern "C" int psph_adapt_h(psph_ctx *c) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (!c->grid_valid) return fail(PSPH_EINVAL, "adapt_h: call psph_neighbors first");
+  const psph_config &k = c->cfg;
+  const bool all = k.list_all || k.dim == 1;
+  const double E = all ? 0.0 : 1.0 / c->grid.inv_e;
+  // the list never reaches past the own bucket or the 4.25 h_i box of list time
+  const double list_reach = std::max(k.list_factor * c->h_max, k.bucket_edge);
+  AdaptPass::Args a{};
+  a.m = c->f.m; a.bc = c->f.bc; a.h = c->f.h; a.done = c->usp;
+  a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.c_smooth = k.c_smooth; a.eta = k.eta;
+  a.ratio = k.ratio_sml_dx; a.thresh = k.thresh; a.pghost = k.pghost_contrib; a.dim = k.dim;
+  if (c->n) HIP_TRY(hipMemsetAsync(c->usp, 0, c->n * 4, c->stream));
+  double hmax = c->h_max;
+  for (int it = 0; it < k.num_loop; it++) {
+    int r = 0;
+    if (!all) {
+      double reach = std::min(k.cutoff * k.c_smooth * hmax, list_reach) * (1 + 1e-9);
+      r = std::max(1, (int)std::ceil(reach / E - 1e-12));
+    }
+    if (c->n) launch_pairs<AdaptPass>(c, make_common(c, r), a);
+    HIP_TRY(hipGetLastError());
+    PSPH_TRY(reduce_hmax(c, &hmax));
+  }
+  PSPH_TRY(check_err(c, "adapt_h"));
+  // the later passes of this step use the new h: widen the cell reach
+  c->h_max = hmax;
+  if (!all) c->r_main = std::max(1, (int)std::ceil(k.cutoff * hmax * (1 + 1e-9) / E - 1e-12));
+  return 0;
+}
+
+extern "C" int psph_rhs(psph_ctx *c, double dt) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (!c->grid_valid) return fail(PSPH_EINVAL, "rhs: call psph_neighbors first");
+  const psph_config &k = c->cfg;
+  uint32_t n = (uint32_t)c->n;
+  // outputs go to spare arrays; non-targets keep their values
+  double *ov[9] = {c->dsp[1], c->dsp[2], c->dsp[3], c->dsp[4], c->dsp[5], c->dsp[6], c->dsp[7], c->dsp[8], c->dsp[9]};
+  double *src[9] = {c->f.vx, c->f.vy, c->f.vz, c->f.e, c->f.p, c->f.c, c->f.T, c->f.gam, c->f.cp};
+  for (int q = 0; q < 9; q++)
+    if (n) HIP_TRY(hipMemcpyAsync(ov[q], src[q], (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+  RhsPass::Args a{};
+  a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.cp = c->f.cp;
+  a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.svx = c->f.svx; a.svy = c->f.svy; a.svz = c->f.svz;
+  a.e = c->f.e; a.xi = c->f.xi; a.bc = c->f.bc;
+  a.o_vx = ov[0]; a.o_vy = ov[1]; a.o_vz = ov[2];
+  a.o_e = ov[3]; a.o_p = ov[4]; a.o_c = ov[5]; a.o_T = ov[6]; a.o_gam = ov[7]; a.o_cp = ov[8];
+  a.dt = dt; a.g = k.g; a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.alpha = k.alpha; a.beta = k.beta;
+  a.ata = k.ata; a.eps_half = k.eps_half; a.prandtl = k.prandtl; a.heat_scale = k.heat_scale;
+  a.lans = k.lans; a.dim = k.dim; a.eos = eos_const(k); a.pairs = c->d_u64 + 2;
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 2, 0ull, 1);
+  if (n) launch_pairs<RhsPass>(c, make_common(c, c->r_main), a);
+  HIP_TRY(hipGetLastError());
+  std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
+  std::swap(c->f.e, c->dsp[4]); std::swap(c->f.p, c->dsp[5]); std::swap(c->f.c, c->dsp[6]);
+  std::swap(c->f.T, c->dsp[7]); std::swap(c->f.gam, c->dsp[8]); std::swap(c->f.cp, c->dsp[9]);
+  HIP_TRY(hipMemcpyAsync(&c->last_pairs, c->d_u64 + 2, 8, hipMemcpyDeviceToHost, c->stream));
+  return check_err(c, "rhs");
+}
+
+extern "C" int psph_density(psph_ctx *c) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (!c->grid_valid) return fail(PSPH_EINVAL, "density: call psph_neighbors first");
+  const psph_config &k = c->cfg;
+  uint32_t n = (uint32_t)c->n;
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(c->dsp[1], c->f.rho, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->dsp[2], c->f.e, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  DensPass::Args a{};
+  a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.e = c->f.e; a.bc = c->f.bc; a.phase = c->f.phase;
+  a.o_rho = c->dsp[1]; a.o_e = c->dsp[2]; a.xi = c->f.xi; a.p = c->f.p; a.c = c->f.c; a.T = c->f.T;
+  a.gam = c->f.gam; a.cp = c->f.cp; a.inv = c->f.inv;
+  a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.e_ratio = k.e_smooth_ratio; a.msfrc_thresh = k.msfrc_thresh;
+  a.pghost = k.pghost_contrib; a.energy_smooth = k.energy_smooth; a.eos = eos_const(k);
+  if (n) launch_pairs<DensPass>(c, make_common(c, c->r_main), a);
+  HIP_TRY(hipGetLastError());
+  std::swap(c->f.rho, c->dsp[1]);
+  std::swap(c->f.e, c->dsp[2]);
+  return check_err(c, "density");
+}
+
+extern "C" int psph_vsmooth(psph_ctx *c) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (!c->grid_valid) return fail(PSPH_EINVAL, "vsmooth: call psph_neighbors first");
+  const psph_config &k = c->cfg;
+  VsmPass::Args a{};
+  a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.vx = c->f.vx;
+  a.vy = k.dim == 3 ? c->f.vy : nullptr; a.vz = k.dim == 3 ? c->f.vz : nullptr;
+  a.bc = c->f.bc; a.svx = c->f.svx; a.svy = k.dim == 3 ? c->f.svy : nullptr; a.svz = k.dim == 3 ? c->f.svz : nullptr;
+  a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2;
+  if (c->n) launch_pairs<VsmPass>(c, make_common(c, c->r_main), a);
+  HIP_TRY(hipGetLastError());
+  return check_err(c, "vsmooth");
+}
+
+static int iferupt(const psph_ctx *c) {     // TimeProps::iferupt (properties.h:126-135)
+  if (c->time < c->cfg.erupt_start) return 0;
+  if (c->time > c->cfg.erupt_end) return 0;
+  return 1;
+}
+
+extern "C" int psph_update_pos(psph_ctx *c, double dt) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  const psph_config &k = c->cfg;
+  uint32_t n = (uint32_t)c->n;
+  if (n)
+    hipLaunchKernelGGL(k_update_pos, blocks(n), 256, 0, c->stream, n, k.dim, k.lans, iferupt(c), dt,
+                       c->f.x, c->f.y, c->f.z, c->f.vx, c->f.vy, c->f.vz, c->f.svx, c->f.svy, c->f.svz,
+                       c->f.bc, c->f.h, c->f.inv, k.dom_lo[2], k.ratio_sml_dx * k.h_air,
+                       k.dom_lo[0], k.dom_lo[1], k.dom_lo[2], k.dom_hi[0], k.dom_hi[1], k.dom_hi[2],
+                       c->d_u64 + 3);
+  HIP_TRY(hipGetLastError());
+  unsigned long long dead = 0;
+  HIP_TRY(hipMemcpyAsync(&dead, c->d_u64 + 3, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->last_removed = (int64_t)dead;
+  c->grid_valid = false;
+  return 0;
+}
+
+extern "C" int psph_inject(psph_ctx *c, double dt, int64_t *added) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (added) *added = 0;
+  c->last_injected = 0;
+  if (c->cfg.dim != 3 || !iferupt(c)) return 0;
+  c->inj.resize(0);
+  PSPH_TRY(inject_host(c->cfg, c->time, dt, c->step, c->inj));
+  size_t m = c->inj.size();
+  if (c->n + m > c->cap)
+    return fail(PSPH_ECAPACITY, "inject: %zu + %zu particles exceed capacity %zu", c->n, m, c->cap);
+  PSPH_TRY(upload(c, c->inj, c->n));
+  c->n += m;
+  c->last_injected = (int64_t)m;
+  if (added) *added = (int64_t)m;
+  c->grid_valid = false;
+  return 0;
+}
+
+extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  const psph_config &k = c->cfg;
+  double dt;
+  PSPH_TRY(psph_timestep(c, &dt));
+  PSPH_TRY(psph_advance_time(c, &dt));
+  PSPH_TRY(psph_neighbors(c));
+  if (k.sml_update_int > 0 && c->step % k.sml_update_int == 0) PSPH_TRY(psph_adapt_h(c));
+  PSPH_TRY(psph_rhs(c, dt));
+  PSPH_TRY(psph_density(c));
+  if (k.lans) PSPH_TRY(psph_vsmooth(c));
+  PSPH_TRY(psph_update_pos(c, dt));
+  int64_t added = 0;
+  PSPH_TRY(psph_inject(c, dt, &added));
+  // TimeProps::ifoutput (properties.h:104-120)
+  if (c->time >= c->next_output || c->time == k.t_end) {
+    c->ioutput++;
+    c->next_output = ((c->ioutput + 1) * k.t_output) / 1.0;
+  }
+  if (out) {
+    out->dt = dt; out->time = c->time; out->step = c->step;
+    out->n_total = (int64_t)c->n - c->last_removed;
+    out->n_real = -1;
+    out->n_injected = added; out->n_removed = c->last_removed;
+    out->n_pairs = (int64_t)c->last_pairs;
+  }
+  return 0;
+}
+
+extern "C" int psph_run(psph_ctx *c, int64_t nsteps, psph_step_out *last) {
+  for (int64_t s = 0; s < nsteps; s++) PSPH_TRY(psph_step(c, last));
+  return 0;
+}
+
+extern "C" int psph_conservation(psph_ctx *c, double out[7]) {
+  if (!c || !out) return fail(PSPH_EINVAL, "null argument");
+  uint32_t n = (uint32_t)c->n;
+  bool d3 = c->cfg.dim == 3;
+  hipLaunchKernelGGL(k_conservation, kRedBlocks, 256, 0, c->stream, n, c->f.bc, c->f.m, c->f.e, c->f.vx,
+                     d3 ? c->f.vy : nullptr, d3 ? c->f.vz : nullptr, c->f.xi, d3 ? c->f.z : nullptr, c->d_part);
+  std::vector<double> part(kRedBlocks * 8);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->d_part, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double acc[8] = {0};
+  for (int b = 0; b < kRedBlocks; b++)
+    for (int k = 0; k < 8; k++) acc[k] += part[b * 8 + k];
+  for (int k = 0; k < 7; k++) out[k] = acc[k];
+  return 0;
+}
+
+extern "C" int psph_diag_height(psph_ctx *c, double *H) {
+  if (!c || !H) return fail(PSPH_EINVAL, "null argument");
+  double cons[7];
+  uint32_t n = (uint32_t)c->n;
+  bool d3 = c->cfg.dim == 3;
+  hipLaunchKernelGGL(k_conservation, kRedBlocks, 256, 0, c->stream, n, c->f.bc, c->f.m, c->f.e, c->f.vx,
+                     d3 ? c->f.vy : nullptr, d3 ? c->f.vz : nullptr, c->f.xi, d3 ? c->f.z : nullptr, c->d_part);
+  std::vector<double> part(kRedBlocks * 8);
+  HIP_TRY(hipMemcpyAsync(part.data(), c->d_part, part.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double num = 0, den = 0;
+  for (int b = 0; b < kRedBlocks; b++) { den += part[b * 8 + 6]; num += part[b * 8 + 7]; }
+  (void)cons;
+  *H = den > 0 ? num / den : 0.0;
+  return 0;
+}
+
+// Neighbour lists for parity checks, produced on the GPU by the same
+// skeleton and membership rule as the physics passes, then mapped to read
+// order and sorted (the reference's list order is bucket-insertion history;
+// the set is what every pass consumes).
+extern "C" int psph_get_neighbors(psph_ctx *c, int64_t *offsets, size_t n_cap, uint32_t *idx, size_t idx_cap) {
+  if (!c || !offsets) return fail(PSPH_EINVAL, "null argument");
+  if (!c->grid_valid) return fail(PSPH_EINVAL, "get_neighbors: call psph_neighbors first");
+  const psph_config &k = c->cfg;
+  uint32_t n = (uint32_t)c->n;
+  int r = 0;
+  if (!(k.list_all || k.dim == 1)) {
+    double E = 1.0 / c->grid.inv_e;
+    double reach = std::max(k.list_factor * c->h_max, k.bucket_edge);
+    r = std::max(1, (int)std::ceil(reach / E - 1e-12));
+  }
+  uint32_t *cnt = nullptr, *off = nullptr, *out = nullptr;
+  PSPH_TRY(dmalloc((void **)&cnt, ((size_t)n + 1) * 4));
+  PSPH_TRY(dmalloc((void **)&off, ((size_t)n + 1) * 4));
+  HIP_TRY(hipMemsetAsync(cnt, 0, ((size_t)n + 1) * 4, c->stream));
+  ListPass::Args a{0, cnt, nullptr, nullptr};
+  if (n) launch_pairs<ListPass>(c, make_common(c, r), a);
+  size_t bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, off, (int)n + 1, c->stream));
+  PSPH_TRY(ensure_cub(c, bytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, cnt, off, (int)n + 1, c->stream));
+  uint32_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, off + n, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  PSPH_TRY(dmalloc((void **)&out, ((size_t)total + 1) * 4));
+  a = ListPass::Args{1, cnt, off, out};
+  if (n) launch_pairs<ListPass>(c, make_common(c, r), a);
+  std::vector<uint32_t> hoff(n + 1), hout(total);
+  HIP_TRY(hipMemcpyAsync(hoff.data(), off, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, c->stream));
+  if (total) HIP_TRY(hipMemcpyAsync(hout.data(), out, (size_t)total * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  hipFree(cnt); hipFree(off); hipFree(out);
+  // device order -> read order
+  HostParticles P;
+  PSPH_TRY(download(c, P));
+  auto perm = canonical_order(c->cfg, P.k0.data(), P.k1.data(), P.k2.data(), n);   // read pos -> dev idx
+  std::vector<uint32_t> rank(n);
+  for (uint32_t q = 0; q < n; q++) rank[perm[q]] = q;
+  if ((size_t)n + 1 > n_cap) return fail(PSPH_ECAPACITY, "get_neighbors: offsets need %u entries", n + 1);
+  size_t o = 0;
+  std::vector<uint32_t> tmp;
+  for (uint32_t q = 0; q < n; q++) {
+    offsets[q] = (int64_t)o;
+    uint32_t d = perm[q];
+    uint32_t b = hoff[d], e = hoff[d + 1];
+    if (P.bc[d] != 100 && P.bc[d] != 99) continue;
+    tmp.assign(hout.begin() + b, hout.begin() + e);
+    for (auto &v : tmp) v = rank[v];
+    std::sort(tmp.begin(), tmp.end());
+    if (idx) {
+      if (o + tmp.size() > idx_cap) return fail(PSPH_ECAPACITY, "get_neighbors: idx buffer holds %zu", idx_cap);
+      std::memcpy(idx + o, tmp.data(), tmp.size() * 4);
+    }
+    o += tmp.size();
+  }
+  offsets[n] = (int64_t)o;
+  return 0;
+}
