@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X SPH particle step on the weak dusty-gas plume.
+
+One "step" is one pass of the reference's per-step hot path (timestep ->
+neighbour search -> [adaptive h every 50 steps] -> momentum/energy RHS ->
+density/mass fraction -> velocity filter -> position update -> inlet) over
+the whole single-GPU domain.  Inputs are synthetic (seeded lattice + 1%
+jitter, standard atmosphere, vent at the origin) and resident in HBM before
+the timed region.  Prints one JSON line (see the repository's driver contract).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "plume-sph_amd"))
+
+import numpy as np  # noqa: E402
+
+import psph  # noqa: E402
+
+METRIC = ("particle-updates/s FP32-pair vs FP64 GPU at 1e-5 rel L1; adjoint/forward time")
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E
+FP64_VECTOR_PEAK_TFS = 78.6      # MI355X FP64 vector
+# Algorithmic HBM bytes per particle-update of the RHS pass: read x,sv,v (72),
+# e,rho,p,c,m,h,T,xi (64), flags (4); write v,e,p,c,T (56) -> 196 B.
+RHS_BYTES_PER_UPDATE = 196.0
+# Algorithmic flops per kernel-support pair in the RHS pass (mul/add/div = 1,
+# FMA = 2, exp = 1).
+RHS_FLOPS_PER_PAIR = 116.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def weak_plume_config(n_real: float, lx=8000.0, ly=8000.0, lz=12000.0):
+    """Weak plume: 8 x 8 x 12 km, N real particles (spacing (V/N)^(1/3)), vent
+    r = 54 m, 135 m/s, 1053 K, erupted-material gas fraction 0.03, p = p(0)."""
+    cfg = psph.default_config(3)
+    cfg.h_air = (lx * ly * lz / n_real) ** (1.0 / 3.0)
+    for k, (lo, hi) in enumerate([(-lx / 2, lx / 2), (-ly / 2, ly / 2), (0.0, lz)]):
+        cfg.dom_lo[k] = lo
+        cfg.dom_hi[k] = hi
+    cfg.vent_z = 0.0
+    cfg.vent_r, cfg.vent_v, cfg.vent_T, cfg.ng0, cfg.vent_p = 54.0, 135.0, 1053.0, 0.03, 101325.0
+    psph.finalize_config(cfg)
+    return cfg
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel, {}).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(n_sample: float, steps: int, spacing: float):
+    """The FP64 CPU restatement (oracle) on a bounded sample of the same
+    workload: same lattice spacing, vent and atmosphere, smaller domain."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    side = (n_sample * spacing ** 3 / 1.5) ** (1.0 / 3.0)
+    cfg = psph.default_config(3)
+    cfg.h_air = spacing
+    for k, (lo, hi) in enumerate([(-side / 2, side / 2), (-side / 2, side / 2), (0.0, 1.5 * side)]):
+        cfg.dom_lo[k] = lo
+        cfg.dom_hi[k] = hi
+    cfg.vent_z = 0.0
+    cfg.vent_r, cfg.vent_v, cfg.vent_T, cfg.ng0, cfg.vent_p = 54.0, 135.0, 1053.0, 0.03, 101325.0
+    psph.finalize_config(cfg)
+    P = psph.gen_plume(cfg, psph.meteo_standard(), 20261204, 0.01)
+    n_real = int(np.count_nonzero(P["bc"] == 100))
+    threads = oracle.lib().orc_set_threads(0)
+    S = oracle.State(cfg, P)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        S.step()
+    wall = time.perf_counter() - t0
+    return {"value": n_real * steps / wall, "unit": "particle-updates/s", "cores": int(threads),
+            "kind": "port",
+            "sample": f"oracle (FP64 C restatement, OpenMP) on a {side / 1000:.2f}x{side / 1000:.2f}x"
+                      f"{1.5 * side / 1000:.2f} km weak-plume box at the same {spacing:.1f} m spacing, "
+                      f"{n_real} real particles, {steps} steps from t=0 ({wall:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=55,
+                    help="untimed steps; >= 50 so the smoothing length has adapted once")
+    ap.add_argument("--particles", type=float, default=4.0e6, help="real particles")
+    ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--cpu-sample", type=float, default=6.0e4, help="real particles in the CPU sample")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="GiB: run only a plain device copy of this size (PMC calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs one GPU's whole domain; multi-GPU sharding is a "
+                                   "design note only (run with --gpus 1)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    if args.copy_calibration > 0:
+        import torch
+        n = int(args.copy_calibration * (1 << 30) // 8)
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
+
+    cfg = weak_plume_config(args.particles)
+    met = psph.meteo_standard()
+    t0 = time.perf_counter()
+    P = psph.gen_plume(cfg, met, args.seed, 0.01)
+    n_total0 = len(P)
+    n_real0 = int(np.count_nonzero(P["bc"] == psph.BC_REAL))
+    log(f"[bench] generated {n_total0} particles ({n_real0} real, spacing {cfg.h_air:.2f} m) "
+        f"in {time.perf_counter() - t0:.1f}s")
+    cap = n_total0 + 400 * (args.warmup + args.steps) + 100000
+    G = psph.Solver(cfg, cap)
+    G.load(P)
+    del P
+    t0 = time.perf_counter()
+    for s in range(args.warmup):
+        G.step()
+    sync()
+    log(f"[bench] {args.warmup} warm-up steps in {time.perf_counter() - t0:.1f}s")
+
+    _, n_real_start = G.count()
+    G.profile(True)
+    pairs = 0
+    sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        o = G.step()
+        pairs += o.n_pairs
+    sync()
+    wall = time.perf_counter() - t0
+    prof = G.profile_read()
+    n_total_end, n_real_end = G.count()
+    t_sim, step = G.time()
+    n_real = 0.5 * (n_real_start + n_real_end)
+    value = n_real * args.steps / wall
+    log(f"[bench] {args.steps} steps in {wall:.3f}s ({1e3 * wall / args.steps:.2f} ms/step), "
+        f"sim time {t_sim:.3f}s after step {step}")
+    for k, (ms, n) in prof.items():
+        log(f"[bench]   {k:11s} {ms:9.2f} ms over {n} launches")
+
+    rhs_ms, rhs_n = prof["rhs"]
+    rhs_s = rhs_ms / max(rhs_n, 1) / 1e3
+    bytes_per_launch = RHS_BYTES_PER_UPDATE * n_real
+    achieved = bytes_per_launch / rhs_s / 1e9
+    traffic = traffic_from_profiles("rhs")
+    roofline = {"kernel": "k_pairs<RhsPass> (momentum/energy RHS)", "bound": "hbm",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "avg_launch_ms": rhs_s * 1e3}
+    flops = RHS_FLOPS_PER_PAIR * (pairs / max(args.steps, 1)) / rhs_s / 1e12
+    roofline_valu = {"bound": "valu_fp64", "achieved": flops, "peak": FP64_VECTOR_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": flops / FP64_VECTOR_PEAK_TFS,
+                     "pairs_per_particle": pairs / max(args.steps, 1) / n_real}
+    G.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_sample, args.cpu_steps, cfg.h_air)
+        except Exception as e:   # the GPU figure stands without the baseline
+            log(f"[bench] cpu baseline failed: {e}")
+            cpu = None
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "weak dusty-gas plume, 8x8x12 km, 4M real particles, FP64 form",
+                   "particles_real": n_real, "particles_total": n_total_end,
+                   "spacing_m": cfg.h_air, "vent": "r=54 m, 135 m/s, 1053 K, gas fraction 0.03",
+                   "kernel": "gaussian-box", "steps_from": step - args.steps, "sim_time_s": t_sim},
+        "roofline": roofline, "roofline_valu": roofline_valu, "cpu_baseline": cpu,
+        "stage_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -238,6 +238,15 @@ int psph_conservation(psph_ctx *ctx, double out[7]);
  * particles, H = sum m xi z / sum m xi.  No reference counterpart. */
 int psph_diag_height(psph_ctx *ctx, double *H);
 
+/* Per-stage GPU time measured with HIP events on the context's stream.
+ * Stage ids: 0 timestep, 1 neighbours (keys+sort+gather+ranges), 2 adapt_h,
+ * 3 rhs, 4 density, 5 vsmooth, 6 update_pos, 7 inject.  psph_profile(ctx, 1)
+ * enables and clears; psph_profile_read returns accumulated ms and the
+ * number of launches of each stage's main kernel. */
+#define PSPH_NSTAGES 8
+int psph_profile(psph_ctx *ctx, int enable);
+int psph_profile_read(psph_ctx *ctx, double ms[PSPH_NSTAGES], int64_t launches[PSPH_NSTAGES]);
+
 const char *psph_last_error(void);
 const char *psph_version(void);
 /* sizeof(psph_config), sizeof(psph_soa), sizeof(psph_step_out) for FFI

@@ -62,16 +62,18 @@ __device__ inline bool contr_dens(int pg, int bc) {
 }
 __device__ inline bool contr_vel(int bc) { return bc == 100 || bc == 99 || bc == 2; }
 
-// |dx_k / h| > cut decided exactly as the reference does (s = dx/h with a
-// correctly rounded division, then compared): far from the cut-off the
-// answer follows from |dx| vs cut*h with a 1e-12 margin; inside the margin
-// the division is performed.
-__device__ inline bool outside_cut(double dx, double h, double cut) {
-  double a = fabs(dx), b = cut * h;
-  if (a > b * (1.0 + 1e-12)) return true;
-  if (a < b * (1.0 - 1e-12)) return false;
-  return fabs(dx / h) > cut;
-}
+// The reference tests abs(s_k) > CUTOFF with s_k = dx_k / h rounded to
+// nearest.  Exactly equivalent and division-free: with u = half an ulp of cut,
+// fl(|dx| / h) > cut  <=>  |dx| / h > cut + u (ties go to cut, whose last
+// mantissa bit is even for cut = 3)  <=>  |dx| - cut h > u h.  The FMA gives
+// round(|dx| - cut h); that difference is a multiple of ~h 2^-53, so it cannot
+// lie within half an ulp of u h without equalling it, and the comparison of
+// the rounded value with the representable u h is exact.  Applied to
+// max_k |dx_k| it decides the whole box.
+struct CutTest {
+  double cut, u;
+  __device__ inline bool outside(double amax, double h) const { return fma(-cut, h, amax) > u * h; }
+};
 
 // Closed-form EOS (particle.cc:528-604).
 struct EosConst { double ng0, cvs, cvg, cva, rg, ra; int lans; };
@@ -235,6 +237,7 @@ struct PairCommon {
   int nx, ny, nz, r;
   int list_all;
   double list_factor;
+  CutTest cut;
   int *err;
 };
 
@@ -286,15 +289,18 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
             for (uint32_t q = 0; q < cnt; q++) {
               double dx[3];
               dx[0] = xi[0] - sx[q];
-              if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
-              if (P::quick_reject(a, t, tile, q, dx)) continue;
-              // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
-              bool in = pc.list_all || srefb[q] == rb_i;
-              if (!in) {
-                in = fabs(dx[0]) <= lf_i;
-                if (D == 3) in = in && fabs(dx[1]) <= lf_i && fabs(dx[2]) <= lf_i;
+              double amax = fabs(dx[0]);
+              if (D == 3) {
+                dx[1] = xi[1] - sy[q];
+                dx[2] = xi[2] - sz[q];
+                amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
               }
-              if (in && P::accept(a, t, tile, q, base + q, i, dx)) mask |= 1ull << q;
+              // outside the kernel box (exact, see CutTest)
+              if (pc.cut.outside(amax, P::hker(a, t, tile, q))) continue;
+              // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
+              const bool in = (pc.list_all || srefb[q] == rb_i || amax <= lf_i) &&
+                              amax <= t.supp && P::extra(a, t, tile, q, base + q, i);
+              if (in) mask |= 1ull << q;
             }
             // phase 2: each lane walks its own hits
             while (mask) {
@@ -333,6 +339,9 @@ __device__ inline bool beyond(const double *dx, double bound) {
 }
 constexpr double kCutMargin = 1.0 + 1e-12;
 
+// Each pass supplies, besides load_target/stage/pair/finish:
+//   bound(q): max |dx_k| beyond which the pair is certainly outside the kernel
+//   hker(q):  smoothing length of the kernel-box test, extra(q): masks.
 // ---------------------------------------------------------------------------
 // RHS pass: mom_engr_update, USE_GSPH==0 (mom_engr_update.cc:30-439)
 // ---------------------------------------------------------------------------
@@ -351,7 +360,7 @@ struct RhsPass {
     double vx[kWave], vy[kWave], vz[kWave];
   };
   struct Tgt {
-    double hi, supp, rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3], rv[3], re;
+    double hi, hb, supp, rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3], rv[3], re;
     unsigned npairs;
     bool bad;
   };
@@ -361,6 +370,7 @@ struct RhsPass {
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i];
+    t.hb = 0.5 * a.cutoff * kCutMargin;
     t.supp = a.cutoff2 * t.hi;
     t.rhoi = a.rho[i];
     t.bad = !(t.rhoi > 0);
@@ -379,78 +389,74 @@ struct RhsPass {
     if (a.lans) { s.vx[l] = a.svx[j]; s.vy[l] = a.svy[j]; s.vz[l] = a.svz[j]; }
     else { s.vx[l] = a.vx[j]; s.vy[l] = a.vy[j]; s.vz[l] = a.vz[j]; }
   }
-  template <int D>
-  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
-    return beyond<D>(dx, 0.5 * (t.hi + s.h[q]) * a.cutoff * kCutMargin);
-  }
-  // j != i, in_support(dx, CUTOFF2 h_i), rho_j > 0 asserted, d_weight != 0
-  template <int D>
-  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t j, uint32_t i,
-                                  const double *dx) {
-    if (j == i) return false;                             // dW(0) = 0
-    if (!in_support(dx, t.supp, D)) return false;
-    if (!(s.rho[q] > 0)) { t.bad = true; return false; }
-    const double h = 0.5 * (t.hi + s.h[q]);
-    for (int k = 0; k < D; k++)
-      if (outside_cut(dx[k], h, a.cutoff)) return false;
-    return true;
-  }
+  // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2); j != i (dW(0) = 0)
+  __device__ static double bound(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return (t.hi + s.h[q]) * t.hb; }
+  __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *dx) {
     t.npairs++;
-    const double hj = s.h[q];
-    const double h = 0.5 * (t.hi + hj);
+    const double rhoj = s.rho[q];
+    if (!(rhoj > 0)) { t.bad = true; return; }            // assert(uvecj[0] > 0)
+    const double h = 0.5 * (t.hi + s.h[q]);
     const double ih = 1.0 / h;
-    double si[3], dist_sq = 0, expt = 0, norm = 1;
+    double si[3], dist_sq = 0, expt = 0;
+#pragma unroll
     for (int k = 0; k < D; k++) {
       si[k] = dx[k] * ih;
       dist_sq += dx[k] * dx[k];
-      norm *= kInvSqrtPi * ih;
       expt += si[k] * si[k];
     }
+    const double nk = kInvSqrtPi * ih;
+    const double norm = (D == 3) ? nk * nk * nk : nk;
     const double W = norm * exp(-expt);
     const double Vj = s.V[q];
     double vij[3];
     vij[0] = t.vel[0] - s.vx[q];
     if (D == 3) { vij[1] = t.vel[1] - s.vy[q]; vij[2] = t.vel[2] - s.vz[q]; }
-    const double rhoab = 0.5 * (t.rhoi + s.rho[q]);
     const double cab = 0.5 * (t.ci + s.c[q]);
-    // art_vis (sph_lib.cc:402-424)
-    double vrab = 0;
-    for (int k = 0; k < D; k++) vrab += dx[k] * vij[k];
+    double vrab = 0, dotvv = 0;
+#pragma unroll
+    for (int k = 0; k < D; k++) { vrab += dx[k] * vij[k]; dotvv += vij[k] * vij[k]; }
+    // art_vis (sph_lib.cc:402-424): Pi = (beta mu - alpha c) mu / rho_ab for
+    // approaching pairs, mu = h v.r / (r^2 + ata h^2)
     double vis = 0;
     if (vrab < 0) {
-      double miu = h * vrab / (dist_sq + a.ata * h * h);
-      vis = (a.beta * miu - a.alpha * cab) * miu / rhoab;
+      const double rhoab = 0.5 * (t.rhoi + rhoj);
+      const double den = dist_sq + a.ata * h * h;
+      const double r = 1.0 / (den * rhoab);
+      const double miu = h * vrab * (rhoab * r);
+      vis = (a.beta * miu - a.alpha * cab) * miu * (den * r);
     }
-    double dotvv = 0;
-    for (int k = 0; k < D; k++) dotvv += vij[k] * vij[k];
     if (a.lans) vis -= a.eps_half * dotvv * Vj;           // SPH_epsilon_mom
     const double mpv = s.m[q] * (s.PV2[q] + t.pvsqi + vis);
+    const double g2 = -2 * ih * W;
     double dw[3];
-    for (int k = 0; k < D; k++) dw[k] = (-2 * si[k] * ih) * W;
+#pragma unroll
+    for (int k = 0; k < D; k++) dw[k] = g2 * si[k];
     double heat = 0;
     if (a.lans && D == 3) {
-      // SPH_epsilon_heat_conductivity (6-arg, sph_lib.cc:1399-1439) and
-      // compute_F (sph_lib.cc:257-281); dW of the heat term equals dW
-      // (HEAT_TRANS_SCALE_RATIO = 1).
+      // k_ij F_ij of SPH_epsilon_heat_conductivity (6-arg, sph_lib.cc:1399-1439)
+      // and compute_F (sph_lib.cc:257-281): 10 eps cp r^2 |v|^2 / (Pr sum|v_k dx_k|)
+      // / (h c) * (dW.dx / r^2); the r^2 factors cancel.
       double dotvr = 0, wr = 0;
+#pragma unroll
       for (int k = 0; k < D; k++) { dotvr += fabs(vij[k] * dx[k]); wr += dw[k] * dx[k]; }
       if (dotvr != 0 && wr != 0) {
         const double cpab = 0.5 * (s.cp[q] + t.cpi);
-        const double kab = 10.0 * a.eps_half * cpab * dist_sq * dotvv / (a.prandtl * dotvr);
-        const double kij = kab / (h * cab);
-        const double F = wr / dist_sq;
-        heat = s.m[q] * Vj * t.Vi * kij * (t.Ti - s.T[q]) * F;
+        const double kF = 10.0 * a.eps_half * cpab * dotvv * wr / (a.prandtl * dotvr * h * cab);
+        heat = s.m[q] * Vj * t.Vi * (t.Ti - s.T[q]) * kF;
       }
     }
     double de = 0;
+#pragma unroll
     for (int k = 0; k < D; k++) {
-      t.rv[k] -= mpv * dw[k];
-      de += 0.5 * (mpv * dw[k]) * vij[k];
+      const double f = mpv * dw[k];
+      t.rv[k] -= f;
+      de += f * vij[k];
     }
-    t.re += de + heat;
+    t.re += 0.5 * de + heat;
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (t.bad) { raise_err(err, PSPH_EINVARIANT, i); return; }
@@ -489,71 +495,60 @@ struct DensPass {
     double cutoff, cutoff2, e_ratio, msfrc_thresh;
     int pghost, energy_smooth;
     EosConst eos;
+    CutTest cut;
   };
   struct Tile {
     double h[kWave], m[kWave], rho[kWave], e[kWave];
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
-  struct Tgt { double hi, supp, r1, r2, wn, en; };
+  struct Tgt { double hi, hb, supp, r1, r2, wn, en; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
-    t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi; t.r1 = t.r2 = t.wn = t.en = 0;
+    t.hi = a.h[i]; t.hb = 0.5 * a.cutoff * kCutMargin; t.supp = a.cutoff2 * t.hi;
+    t.r1 = t.r2 = t.wn = t.en = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
     s.h[l] = a.h[j]; s.m[l] = a.m[j]; s.rho[l] = a.rho[j]; s.e[l] = a.e[j];
     int ph = a.phase[j];
     s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
   }
-  template <int D>
-  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
-    return !s.ok[q] || beyond<D>(dx, 0.5 * (t.hi + s.h[q]) * a.cutoff * kCutMargin);
-  }
-  // contr_dens and phase, in_support(ds, CUTOFF2 h_i), weight != 0 at h_ij.
-  // The energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
-  template <int D>
-  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                  const double *ds) {
-    if (!in_support(ds, t.supp, D)) return false;
-    const double hij = 0.5 * (t.hi + s.h[q]);
-    for (int k = 0; k < D; k++)
-      if (outside_cut(ds[k], hij, a.cutoff)) return false;
-    return true;
-  }
+  // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
+  // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
+  __device__ static double bound(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return (t.hi + s.h[q]) * t.hb; }
+  __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds) {
     const double hij = 0.5 * (t.hi + s.h[q]);
     const double ih = 1.0 / hij;
     const double mj = s.m[q];
-    if (a.energy_smooth) {
-      // weight(4 ds / h_ij, h_ij / 4): support |ds| <= 0.75 h_ij
-      bool out = false;
-      for (int k = 0; k < D; k++) out = out || outside_cut(a.e_ratio * ds[k], hij, a.cutoff);
-      if (!out) {
-        const double ihe = 1.0 / (hij / a.e_ratio);
-        double ne = 1, ee = 0;
-        for (int k = 0; k < D; k++) {
-          double se = a.e_ratio * ds[k] * ih;
-          ne *= kInvSqrtPi * ihe;
-          ee += se * se;
-        }
-        const double wm_e = ne * exp(-ee) * mj;
-        const double rj = s.rho[q];
-        t.wn += wm_e / rj;
-        t.en += wm_e * s.e[q] / rj;
-      }
-    }
-    double expt = 0, norm = 1;
+    double expt = 0, amax = 0;
+#pragma unroll
     for (int k = 0; k < D; k++) {
       double sk = ds[k] * ih;
-      norm *= kInvSqrtPi * ih;
       expt += sk * sk;
+      amax = fmax(amax, fabs(ds[k]));
     }
+    const double nk = kInvSqrtPi * ih;
+    const double norm = (D == 3) ? nk * nk * nk : nk;
     const double wm = norm * exp(-expt) * mj;
     if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
+    if (a.energy_smooth) {
+      // weight(4 ds / h_ij, h_ij / 4) (HAVE_ENERGY_SMOOTH 1)
+      if (!a.cut.outside(a.e_ratio * amax, hij)) {
+        const double nke = kInvSqrtPi * (a.e_ratio * ih);
+        const double ne = (D == 3) ? nke * nke * nke : nke;
+        const double e2 = a.e_ratio * a.e_ratio;
+        const double wm_e = ne * exp(-(e2 * expt)) * mj;
+        const double vr = wm_e / s.rho[q];
+        t.wn += vr;
+        t.en += vr * s.e[q];
+      }
+    }
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     double density = 0.0;
@@ -590,14 +585,15 @@ struct VsmPass {
     double mr[kWave], vx[kWave], vy[kWave], vz[kWave];
     int ok[kWave];
   };
-  struct Tgt { double hi, ih, bound, supp, norm, v[3], dv[3], wn; };
+  struct Tgt { double hi, ih, bnd, supp, norm, v[3], dv[3], wn; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
-    t.hi = a.h[i]; t.ih = 1.0 / t.hi; t.bound = a.cutoff * t.hi * kCutMargin; t.supp = a.cutoff2 * t.hi;
-    t.norm = kInvSqrtPi * t.ih;   // per-axis factor of the kernel norm
+    t.hi = a.h[i]; t.ih = 1.0 / t.hi; t.bnd = a.cutoff * t.hi * kCutMargin; t.supp = a.cutoff2 * t.hi;
+    const double nk = kInvSqrtPi * t.ih;
+    t.norm = a.vy ? nk * nk * nk : nk;
     t.v[0] = a.vx[i]; t.v[1] = a.vy ? a.vy[i] : 0; t.v[2] = a.vz ? a.vz[i] : 0;
     t.dv[0] = t.dv[1] = t.dv[2] = 0; t.wn = 0;
   }
@@ -608,28 +604,19 @@ struct VsmPass {
     s.vz[l] = a.vz ? a.vz[j] : 0;
     s.ok[l] = contr_vel(a.bc[j]);
   }
-  template <int D>
-  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
-    return !s.ok[q] || beyond<D>(dx, t.bound);
-  }
-  template <int D>
-  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                  const double *ds) {
-    if (!in_support(ds, t.supp, D)) return false;
-    for (int k = 0; k < D; k++)
-      if (outside_cut(ds[k], t.hi, a.cutoff)) return false;
-    return true;
-  }
+  __device__ static double bound(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.bnd; }
+  __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds) {
-    double expt = 0, norm = 1;
+    double expt = 0;
+#pragma unroll
     for (int k = 0; k < D; k++) {
       double sk = ds[k] * t.ih;
-      norm *= t.norm;
       expt += sk * sk;
     }
-    const double mvw = norm * exp(-expt) * s.mr[q];
+    const double mvw = t.norm * exp(-expt) * s.mr[q];
     t.dv[0] += mvw * (s.vx[q] - t.v[0]);
     if (D == 3) {
       t.dv[1] += mvw * (s.vy[q] - t.v[1]);
@@ -639,8 +626,9 @@ struct VsmPass {
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
-    a.svx[i] = t.v[0] + t.dv[0] / t.wn;
-    if (a.svy) { a.svy[i] = t.v[1] + t.dv[1] / t.wn; a.svz[i] = t.v[2] + t.dv[2] / t.wn; }
+    const double iw = 1.0 / t.wn;
+    a.svx[i] = t.v[0] + t.dv[0] * iw;
+    if (a.svy) { a.svy[i] = t.v[1] + t.dv[1] * iw; a.svz[i] = t.v[2] + t.dv[2] * iw; }
   }
 };
 
@@ -662,13 +650,15 @@ struct AdaptPass {
     double m[kWave];
     int ok[kWave];
   };
-  struct Tgt { double mi, hcur, Hs, iHs, bound, supp, rho_star; bool done; };
+  struct Tgt { double mi, hcur, Hs, iHs, bnd, supp, norm, rho_star; bool done; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &a, Tgt &t, int) {
     t.Hs = t.hcur * a.c_smooth;
     t.iHs = 1.0 / t.Hs;
-    t.bound = a.cutoff * t.Hs * kCutMargin;
+    t.bnd = a.cutoff * t.Hs * kCutMargin;
     t.supp = a.cutoff2 * t.Hs;
+    const double nk = kInvSqrtPi * t.iHs;
+    t.norm = a.dim == 3 ? nk * nk * nk : nk;
     t.rho_star = 0;
   }
   __device__ static bool want(const Tgt &t) { return !t.done; }
@@ -679,28 +669,19 @@ struct AdaptPass {
     s.m[l] = a.m[j];
     s.ok[l] = contr_dens(a.pghost, a.bc[j]);
   }
-  template <int D>
-  __device__ static bool reject_d(const Args &a, const Tgt &t, const Tile &s, uint32_t q, const double *dx) {
-    return !s.ok[q] || beyond<D>(dx, t.bound);
-  }
-  template <int D>
-  __device__ static bool accept_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                  const double *dx) {
-    if (!in_support(dx, t.supp, D)) return false;
-    for (int k = 0; k < D; k++)
-      if (outside_cut(dx[k], t.Hs, a.cutoff)) return false;
-    return true;
-  }
+  __device__ static double bound(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.bnd; }
+  __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.Hs; }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *dx) {
-    double expt = 0, norm = 1;
+    double expt = 0;
+#pragma unroll
     for (int k = 0; k < D; k++) {
       double sk = dx[k] * t.iHs;
-      norm *= kInvSqrtPi * t.iHs;
       expt += sk * sk;
     }
-    t.rho_star += norm * exp(-expt) * s.m[q];
+    t.rho_star += t.norm * exp(-expt) * s.m[q];
   }
   __device__ static void end_pass(const Args &a, Tgt &t, int, uint32_t, bool active) {
     if (!active || t.done) return;
@@ -726,9 +707,10 @@ struct ListPass {
     uint32_t *count;
     const uint32_t *offset;
     uint32_t *out;
+    double cutoff;
   };
   struct Tile { int dummy; };
-  struct Tgt { uint32_t n, o; };
+  struct Tgt { uint32_t n, o; double supp; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
@@ -736,17 +718,12 @@ struct ListPass {
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.n = 0;
     t.o = a.mode ? a.offset[i] : 0;
+    t.supp = HUGE_VAL;
   }
   __device__ static void stage(const Args &, uint32_t, Tile &, int) {}
-  template <int D>
-  __device__ static bool reject_d(const Args &, const Tgt &, const Tile &, uint32_t, const double *) {
-    return false;
-  }
-  template <int D>
-  __device__ static bool accept_d(const Args &, Tgt &, const Tile &, uint32_t, uint32_t, uint32_t,
-                                  const double *) {
-    return true;
-  }
+  __device__ static double bound(const Args &, const Tgt &, const Tile &, uint32_t) { return HUGE_VAL; }
+  __device__ static double hker(const Args &, const Tgt &, const Tile &, uint32_t) { return HUGE_VAL; }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t, uint32_t) { return true; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &, uint32_t, uint32_t j, uint32_t,
                                 const double *) {
@@ -758,17 +735,9 @@ struct ListPass {
   }
 };
 
-// Wrap a pass so the skeleton can call P::pair with the dimension bound.
+// Binds the dimension into the pass's pair routine for the skeleton.
 template <int D, class P>
 struct Bound : P {
-  __device__ static bool quick_reject(const typename P::Args &a, const typename P::Tgt &t,
-                                      const typename P::Tile &s, uint32_t q, const double *dx) {
-    return P::template reject_d<D>(a, t, s, q, dx);
-  }
-  __device__ static bool accept(const typename P::Args &a, typename P::Tgt &t, const typename P::Tile &s,
-                                uint32_t q, uint32_t j, uint32_t i, const double *dx) {
-    return P::template accept_d<D>(a, t, s, q, j, i, dx);
-  }
   __device__ static void pair(const typename P::Args &a, typename P::Tgt &t, const typename P::Tile &s,
                              uint32_t q, uint32_t j, uint32_t i, const double *dx) {
     P::template pair_d<D>(a, t, s, q, j, i, dx);
@@ -892,6 +861,32 @@ struct psph_ctx {
   int64_t last_injected = 0, last_removed = 0;
   unsigned long long last_pairs = 0;
   HostParticles inj;
+  // per-stage HIP-event timing
+  bool prof = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double prof_ms[PSPH_NSTAGES] = {};
+  int64_t prof_n[PSPH_NSTAGES] = {};
+};
+
+// Brackets the main kernel(s) of a stage with events on the context stream.
+struct StageTimer {
+  psph_ctx *c;
+  int id;
+  StageTimer(psph_ctx *ctx, int stage) : c(ctx), id(stage) {
+    if (c->prof) (void)hipEventRecord(c->ev0, c->stream);
+  }
+  void stop() {
+    if (!c->prof || id < 0) return;
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) {
+      c->prof_ms[id] += ms;
+      c->prof_n[id] += 1;
+    }
+    id = -1;
+  }
+  ~StageTimer() { stop(); }
 };
 
 static int dmalloc(void **p, size_t bytes) {
@@ -928,6 +923,13 @@ extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **o
   if (cfg->precision != PSPH_FP64) return fail(PSPH_ENOTSUP, "only the FP64 form is built in this library version");
   if (cfg->kernel != PSPH_KERNEL_GAUSS_BOX) return fail(PSPH_ENOTSUP, "only the Gaussian box kernel is built");
   if (!(cfg->h_air > 0)) return fail(PSPH_EINVAL, "h_air must be > 0");
+  {
+    uint64_t bits;
+    std::memcpy(&bits, &cfg->cutoff, 8);
+    if (!(cfg->cutoff > 0) || (bits & 1u))
+      return fail(PSPH_ENOTSUP, "cutoff %g: the exact kernel-box test needs a positive cut-off with an even "
+                                "last significand bit", cfg->cutoff);
+  }
   if (cfg->dim == 3 && !(cfg->bucket_edge > 0)) return fail(PSPH_EINVAL, "call psph_config_domain first");
   if (capacity < 64) capacity = 64;
   if (capacity > 0xffffff00u) return fail(PSPH_EINVAL, "capacity above 2^32 particles");
@@ -973,6 +975,8 @@ extern "C" int psph_destroy(psph_ctx *c) {
                   c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->crow, c->cs,
                   c->row_chunks, c->row_off, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part};
   for (auto p : ptrs) if (p) hipFree(p);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -1060,8 +1064,10 @@ extern "C" int psph_read(psph_ctx *c, psph_soa *host, size_t cap, size_t *n_out)
 extern "C" int psph_timestep(psph_ctx *c, double *dt_out) {
   if (!c || !dt_out) return fail(PSPH_EINVAL, "null argument");
   unsigned long long init = (unsigned long long)0x7FF0000000000000ull;   // +inf bits
+  StageTimer tm(c, 0);
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64, init, 1);
   if (c->n) hipLaunchKernelGGL(k_dt_min, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->f.c, c->d_u64, c->d_err);
+  tm.stop();
   unsigned long long bits;
   HIP_TRY(hipMemcpyAsync(&bits, c->d_u64, 8, hipMemcpyDeviceToHost, c->stream));
   PSPH_TRY(check_err(c, "timestep"));
@@ -1151,6 +1157,7 @@ static int build_grid(psph_ctx *c) {
 extern "C" int psph_neighbors(psph_ctx *c) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   PSPH_TRY(build_grid(c));
+  StageTimer tm(c, 1);
   const GridDesc &g = c->grid;
   uint32_t n = (uint32_t)c->n;
   hipStream_t s = c->stream;
@@ -1209,8 +1216,16 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   hipLaunchKernelGGL(k_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, c->cs, c->tpos, c->row_off,
                      c->cfirst, c->ccount, c->crow, c->nchunks);
   HIP_TRY(hipGetLastError());
+  tm.stop();
   c->grid_valid = true;
   return 0;
+}
+
+// half an ulp of the cut-off; the test above assumes its mantissa is even
+static CutTest cut_test(double cut) {
+  int e;
+  std::frexp(cut, &e);                       // cut = f 2^e, f in [0.5, 1)
+  return CutTest{cut, std::ldexp(1.0, e - 54)};
 }
 
 static PairCommon make_common(psph_ctx *c, int r) {
@@ -1222,6 +1237,7 @@ static PairCommon make_common(psph_ctx *c, int r) {
   pc.nx = c->grid.nx; pc.ny = c->grid.ny; pc.nz = c->grid.nz; pc.r = r;
   pc.list_all = (c->cfg.list_all || c->cfg.dim == 1) ? 1 : 0;
   pc.list_factor = c->cfg.list_factor;
+  pc.cut = cut_test(c->cfg.cutoff);
   pc.err = c->d_err;
   return pc;
 }
@@ -1273,7 +1289,10 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
       double reach = std::min(k.cutoff * k.c_smooth * hmax, list_reach) * (1 + 1e-9);
       r = std::max(1, (int)std::ceil(reach / E - 1e-12));
     }
-    if (c->n) launch_pairs<AdaptPass>(c, make_common(c, r), a);
+    {
+      StageTimer tm(c, 2);
+      if (c->n) launch_pairs<AdaptPass>(c, make_common(c, r), a);
+    }
     HIP_TRY(hipGetLastError());
     PSPH_TRY(reduce_hmax(c, &hmax));
   }
@@ -1304,7 +1323,10 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   a.ata = k.ata; a.eps_half = k.eps_half; a.prandtl = k.prandtl; a.heat_scale = k.heat_scale;
   a.lans = k.lans; a.dim = k.dim; a.eos = eos_const(k); a.pairs = c->d_u64 + 2;
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 2, 0ull, 1);
-  if (n) launch_pairs<RhsPass>(c, make_common(c, c->r_main), a);
+  {
+    StageTimer tm(c, 3);
+    if (n) launch_pairs<RhsPass>(c, make_common(c, c->r_main), a);
+  }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
   std::swap(c->f.e, c->dsp[4]); std::swap(c->f.p, c->dsp[5]); std::swap(c->f.c, c->dsp[6]);
@@ -1328,7 +1350,11 @@ extern "C" int psph_density(psph_ctx *c) {
   a.gam = c->f.gam; a.cp = c->f.cp; a.inv = c->f.inv;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.e_ratio = k.e_smooth_ratio; a.msfrc_thresh = k.msfrc_thresh;
   a.pghost = k.pghost_contrib; a.energy_smooth = k.energy_smooth; a.eos = eos_const(k);
-  if (n) launch_pairs<DensPass>(c, make_common(c, c->r_main), a);
+  a.cut = cut_test(k.cutoff);
+  {
+    StageTimer tm(c, 4);
+    if (n) launch_pairs<DensPass>(c, make_common(c, c->r_main), a);
+  }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.rho, c->dsp[1]);
   std::swap(c->f.e, c->dsp[2]);
@@ -1344,7 +1370,10 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   a.vy = k.dim == 3 ? c->f.vy : nullptr; a.vz = k.dim == 3 ? c->f.vz : nullptr;
   a.bc = c->f.bc; a.svx = c->f.svx; a.svy = k.dim == 3 ? c->f.svy : nullptr; a.svz = k.dim == 3 ? c->f.svz : nullptr;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2;
-  if (c->n) launch_pairs<VsmPass>(c, make_common(c, c->r_main), a);
+  {
+    StageTimer tm(c, 5);
+    if (c->n) launch_pairs<VsmPass>(c, make_common(c, c->r_main), a);
+  }
   HIP_TRY(hipGetLastError());
   return check_err(c, "vsmooth");
 }
@@ -1359,12 +1388,14 @@ extern "C" int psph_update_pos(psph_ctx *c, double dt) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   const psph_config &k = c->cfg;
   uint32_t n = (uint32_t)c->n;
+  StageTimer tm(c, 6);
   if (n)
     hipLaunchKernelGGL(k_update_pos, blocks(n), 256, 0, c->stream, n, k.dim, k.lans, iferupt(c), dt,
                        c->f.x, c->f.y, c->f.z, c->f.vx, c->f.vy, c->f.vz, c->f.svx, c->f.svy, c->f.svz,
                        c->f.bc, c->f.h, c->f.inv, k.dom_lo[2], k.ratio_sml_dx * k.h_air,
                        k.dom_lo[0], k.dom_lo[1], k.dom_lo[2], k.dom_hi[0], k.dom_hi[1], k.dom_hi[2],
                        c->d_u64 + 3);
+  tm.stop();
   HIP_TRY(hipGetLastError());
   unsigned long long dead = 0;
   HIP_TRY(hipMemcpyAsync(&dead, c->d_u64 + 3, 8, hipMemcpyDeviceToHost, c->stream));
@@ -1384,7 +1415,10 @@ extern "C" int psph_inject(psph_ctx *c, double dt, int64_t *added) {
   size_t m = c->inj.size();
   if (c->n + m > c->cap)
     return fail(PSPH_ECAPACITY, "inject: %zu + %zu particles exceed capacity %zu", c->n, m, c->cap);
-  PSPH_TRY(upload(c, c->inj, c->n));
+  {
+    StageTimer tm(c, 7);
+    PSPH_TRY(upload(c, c->inj, c->n));
+  }
   c->n += m;
   c->last_injected = (int64_t)m;
   if (added) *added = (int64_t)m;
@@ -1478,7 +1512,7 @@ extern "C" int psph_get_neighbors(psph_ctx *c, int64_t *offsets, size_t n_cap, u
   PSPH_TRY(dmalloc((void **)&cnt, ((size_t)n + 1) * 4));
   PSPH_TRY(dmalloc((void **)&off, ((size_t)n + 1) * 4));
   HIP_TRY(hipMemsetAsync(cnt, 0, ((size_t)n + 1) * 4, c->stream));
-  ListPass::Args a{0, cnt, nullptr, nullptr};
+  ListPass::Args a{0, cnt, nullptr, nullptr, k.cutoff};
   if (n) launch_pairs<ListPass>(c, make_common(c, r), a);
   size_t bytes = 0;
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, off, (int)n + 1, c->stream));
@@ -1488,7 +1522,7 @@ extern "C" int psph_get_neighbors(psph_ctx *c, int64_t *offsets, size_t n_cap, u
   HIP_TRY(hipMemcpyAsync(&total, off + n, 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   PSPH_TRY(dmalloc((void **)&out, ((size_t)total + 1) * 4));
-  a = ListPass::Args{1, cnt, off, out};
+  a = ListPass::Args{1, cnt, off, out, k.cutoff};
   if (n) launch_pairs<ListPass>(c, make_common(c, r), a);
   std::vector<uint32_t> hoff(n + 1), hout(total);
   HIP_TRY(hipMemcpyAsync(hoff.data(), off, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1519,5 +1553,25 @@ extern "C" int psph_get_neighbors(psph_ctx *c, int64_t *offsets, size_t n_cap, u
     o += tmp.size();
   }
   offsets[n] = (int64_t)o;
+  return 0;
+}
+
+extern "C" int psph_profile(psph_ctx *c, int enable) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (enable && !c->ev0) {
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+  }
+  c->prof = enable != 0;
+  for (int k = 0; k < PSPH_NSTAGES; k++) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
+  return 0;
+}
+
+extern "C" int psph_profile_read(psph_ctx *c, double ms[PSPH_NSTAGES], int64_t launches[PSPH_NSTAGES]) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  for (int k = 0; k < PSPH_NSTAGES; k++) {
+    if (ms) ms[k] = c->prof_ms[k];
+    if (launches) launches[k] = c->prof_n[k];
+  }
   return 0;
 }

@@ -179,6 +179,8 @@ def lib() -> C.CDLL:
         "psph_last_error": (C.c_char_p, []),
         "psph_version": (C.c_char_p, []),
         "psph_abi_sizeof": (C.c_size_t, [C.c_int]),
+        "psph_profile": (C.c_int, [P, C.c_int]),
+        "psph_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -397,6 +399,19 @@ class Solver:
         out = np.zeros(7)
         check(lib().psph_conservation(self._ctx, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
+
+    STAGES = ["timestep", "neighbors", "adapt_h", "rhs", "density", "vsmooth", "update_pos", "inject"]
+
+    def profile(self, enable: bool = True):
+        """Enable (and clear) per-stage HIP-event timing on the library's stream."""
+        check(lib().psph_profile(self._ctx, int(enable)))
+
+    def profile_read(self) -> dict:
+        ms = np.zeros(8)
+        n = np.zeros(8, np.int64)
+        check(lib().psph_profile_read(self._ctx, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                      n.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 
     def diag_height(self) -> float:
         H = C.c_double()
