@@ -28,6 +28,7 @@ import psph  # noqa: E402
 METRIC = ("particle-updates/s FP32-pair vs FP64 GPU at 1e-5 rel L1; adjoint/forward time")
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E
 FP64_VECTOR_PEAK_TFS = 78.6      # MI355X FP64 vector
+FP32_VECTOR_PEAK_TFS = 157.3     # MI355X FP32 vector
 # Algorithmic HBM bytes per particle-update of the RHS pass: read x,sv,v (72),
 # e,rho,p,c,m,h,T,xi (64), flags (4); write v,e,p,c,T (56) -> 196 B.
 RHS_BYTES_PER_UPDATE = 196.0
@@ -113,6 +114,7 @@ def main():
                     help="untimed steps; >= 50 so the smoothing length has adapted once")
     ap.add_argument("--particles", type=float, default=4.0e6, help="real particles")
     ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--precision", choices=["fp64", "fp32-pair"], default="fp64")
     ap.add_argument("--cpu-sample", type=float, default=6.0e4, help="real particles in the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -137,6 +139,8 @@ def main():
         return
 
     cfg = weak_plume_config(args.particles)
+    if args.precision == "fp32-pair":
+        cfg.precision = psph.PSPH_FP32_PAIR
     met = psph.meteo_standard()
     t0 = time.perf_counter()
     P = psph.gen_plume(cfg, met, args.seed, 0.01)
@@ -184,8 +188,9 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_ms": rhs_s * 1e3}
     flops = RHS_FLOPS_PER_PAIR * (pairs / max(args.steps, 1)) / rhs_s / 1e12
-    roofline_valu = {"bound": "valu_fp64", "achieved": flops, "peak": FP64_VECTOR_PEAK_TFS,
-                     "unit": "TFLOP/s", "frac": flops / FP64_VECTOR_PEAK_TFS,
+    vpeak = FP64_VECTOR_PEAK_TFS if args.precision == "fp64" else FP32_VECTOR_PEAK_TFS
+    roofline_valu = {"bound": "valu_fp64" if args.precision == "fp64" else "valu_fp32",
+                     "achieved": flops, "peak": vpeak, "unit": "TFLOP/s", "frac": flops / vpeak,
                      "pairs_per_particle": pairs / max(args.steps, 1) / n_real}
     G.close()
 
@@ -200,9 +205,11 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-pair+f64",
         "data": "synthetic",
-        "config": {"workload": "weak dusty-gas plume, 8x8x12 km, 4M real particles, FP64 form",
+        "config": {"workload": f"weak dusty-gas plume, 8x8x12 km, {args.particles / 1e6:g}M real particles, "
+                               f"{'FP64 form' if args.precision == 'fp64' else 'FP32-pair form'}",
                    "particles_real": n_real, "particles_total": n_total_end,
                    "spacing_m": cfg.h_air, "vent": "r=54 m, 135 m/s, 1053 K, gas fraction 0.03",
                    "kernel": "gaussian-box", "steps_from": step - args.steps, "sim_time_s": t_sim},

@@ -340,27 +340,43 @@ __device__ inline bool beyond(const double *dx, double bound) {
 constexpr double kCutMargin = 1.0 + 1e-12;
 
 // Each pass supplies, besides load_target/stage/pair/finish:
-//   bound(q): max |dx_k| beyond which the pair is certainly outside the kernel
 //   hker(q):  smoothing length of the kernel-box test, extra(q): masks.
+// The pair arithmetic type R is double (FP64 form) or float (FP32-pair form):
+// the pair predicate (phase 1) is FP64 in both, so both forms visit exactly
+// the same pairs; accumulators, integration and the EOS stay FP64.
+// ---------------------------------------------------------------------------
+template <class R> __device__ inline R rexp(R x);
+template <> __device__ inline double rexp<double>(double x) { return exp(x); }
+template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
+template <class R> __device__ inline R rrcp(R x);
+template <> __device__ inline double rrcp<double>(double x) { return 1.0 / x; }
+template <> __device__ inline float rrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <class R> __device__ inline R rabs(R x) { return x < R(0) ? -x : x; }
+
 // ---------------------------------------------------------------------------
 // RHS pass: mom_engr_update, USE_GSPH==0 (mom_engr_update.cc:30-439)
 // ---------------------------------------------------------------------------
+struct RhsArgs {
+  const double *h, *m, *rho, *p, *c, *T, *cp, *vx, *vy, *vz, *svx, *svy, *svz, *e, *xi;
+  const int32_t *bc;
+  double *o_vx, *o_vy, *o_vz, *o_e, *o_p, *o_c, *o_T, *o_gam, *o_cp;
+  double dt, g, cutoff, cutoff2, alpha, beta, ata, eps_half, prandtl, heat_scale;
+  int lans, dim;
+  EosConst eos;
+  unsigned long long *pairs;
+};
+
+template <class R>
 struct RhsPass {
-  struct Args {
-    const double *h, *m, *rho, *p, *c, *T, *cp, *vx, *vy, *vz, *svx, *svy, *svz, *e, *xi;
-    const int32_t *bc;
-    double *o_vx, *o_vy, *o_vz, *o_e, *o_p, *o_c, *o_T, *o_gam, *o_cp;
-    double dt, g, cutoff, cutoff2, alpha, beta, ata, eps_half, prandtl, heat_scale;
-    int lans, dim;
-    EosConst eos;
-    unsigned long long *pairs;
-  };
+  using Args = RhsArgs;
   struct Tile {
-    double h[kWave], m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
-    double vx[kWave], vy[kWave], vz[kWave];
+    double h[kWave];
+    R m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
+    R vx[kWave], vy[kWave], vz[kWave];
   };
   struct Tgt {
-    double hi, hb, supp, rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3], rv[3], re;
+    double hi, supp, rv[3], re;
+    R rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3];
     unsigned npairs;
     bool bad;
   };
@@ -370,110 +386,105 @@ struct RhsPass {
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i];
-    t.hb = 0.5 * a.cutoff * kCutMargin;
     t.supp = a.cutoff2 * t.hi;
-    t.rhoi = a.rho[i];
-    t.bad = !(t.rhoi > 0);
-    t.Vi = 1.0 / t.rhoi;
-    t.pvsqi = a.p[i] * t.Vi * t.Vi;
-    t.ci = a.c[i]; t.Ti = a.T[i]; t.cpi = a.cp[i];
-    if (a.lans) { t.vel[0] = a.svx[i]; t.vel[1] = a.svy[i]; t.vel[2] = a.svz[i]; }
-    else { t.vel[0] = a.vx[i]; t.vel[1] = a.vy[i]; t.vel[2] = a.vz[i]; }
+    const double rho = a.rho[i];
+    t.bad = !(rho > 0);
+    const double Vi = 1.0 / rho;
+    t.rhoi = (R)rho; t.Vi = (R)Vi; t.pvsqi = (R)(a.p[i] * Vi * Vi);
+    t.ci = (R)a.c[i]; t.Ti = (R)a.T[i]; t.cpi = (R)a.cp[i];
+    if (a.lans) { t.vel[0] = (R)a.svx[i]; t.vel[1] = (R)a.svy[i]; t.vel[2] = (R)a.svz[i]; }
+    else { t.vel[0] = (R)a.vx[i]; t.vel[1] = (R)a.vy[i]; t.vel[2] = (R)a.vz[i]; }
     t.rv[0] = t.rv[1] = t.rv[2] = 0; t.re = 0; t.npairs = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    double rj = a.rho[j];
-    double V = 1.0 / rj;
-    s.h[l] = a.h[j]; s.m[l] = a.m[j]; s.V[l] = V; s.PV2[l] = a.p[j] * V * V; s.rho[l] = rj;
-    s.c[l] = a.c[j]; s.T[l] = a.T[j]; s.cp[l] = a.cp[j];
-    if (a.lans) { s.vx[l] = a.svx[j]; s.vy[l] = a.svy[j]; s.vz[l] = a.svz[j]; }
-    else { s.vx[l] = a.vx[j]; s.vy[l] = a.vy[j]; s.vz[l] = a.vz[j]; }
+    const double rj = a.rho[j];
+    const double V = 1.0 / rj;
+    s.h[l] = a.h[j]; s.m[l] = (R)a.m[j]; s.V[l] = (R)V; s.PV2[l] = (R)(a.p[j] * V * V); s.rho[l] = (R)rj;
+    s.c[l] = (R)a.c[j]; s.T[l] = (R)a.T[j]; s.cp[l] = (R)a.cp[j];
+    if (a.lans) { s.vx[l] = (R)a.svx[j]; s.vy[l] = (R)a.svy[j]; s.vz[l] = (R)a.svz[j]; }
+    else { s.vx[l] = (R)a.vx[j]; s.vy[l] = (R)a.vy[j]; s.vz[l] = (R)a.vz[j]; }
   }
   // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2); j != i (dW(0) = 0)
-  __device__ static double bound(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return (t.hi + s.h[q]) * t.hb; }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                const double *dx) {
+                                const double *dxd) {
     t.npairs++;
-    const double rhoj = s.rho[q];
-    if (!(rhoj > 0)) { t.bad = true; return; }            // assert(uvecj[0] > 0)
-    const double h = 0.5 * (t.hi + s.h[q]);
-    const double ih = 1.0 / h;
-    double si[3], dist_sq = 0, expt = 0;
+    const R rhoj = s.rho[q];
+    if (!(rhoj > R(0))) { t.bad = true; return; }       // assert(uvecj[0] > 0)
+    const R h = (R)(0.5 * (t.hi + s.h[q]));
+    const R ih = rrcp<R>(h);
+    R dx[3], si[3], dist_sq = 0, expt = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
+      dx[k] = (R)dxd[k];
       si[k] = dx[k] * ih;
       dist_sq += dx[k] * dx[k];
       expt += si[k] * si[k];
     }
-    const double nk = kInvSqrtPi * ih;
-    const double norm = (D == 3) ? nk * nk * nk : nk;
-    const double W = norm * exp(-expt);
-    const double Vj = s.V[q];
-    double vij[3];
+    const R nk = R(kInvSqrtPi) * ih;
+    const R norm = (D == 3) ? nk * nk * nk : nk;
+    const R W = norm * rexp<R>(-expt);
+    const R Vj = s.V[q];
+    R vij[3];
     vij[0] = t.vel[0] - s.vx[q];
     if (D == 3) { vij[1] = t.vel[1] - s.vy[q]; vij[2] = t.vel[2] - s.vz[q]; }
-    const double cab = 0.5 * (t.ci + s.c[q]);
-    double vrab = 0, dotvv = 0;
+    const R cab = R(0.5) * (t.ci + s.c[q]);
+    R vrab = 0, dotvv = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) { vrab += dx[k] * vij[k]; dotvv += vij[k] * vij[k]; }
     // art_vis (sph_lib.cc:402-424): Pi = (beta mu - alpha c) mu / rho_ab for
     // approaching pairs, mu = h v.r / (r^2 + ata h^2)
-    double vis = 0;
-    if (vrab < 0) {
-      const double rhoab = 0.5 * (t.rhoi + rhoj);
-      const double den = dist_sq + a.ata * h * h;
-      const double r = 1.0 / (den * rhoab);
-      const double miu = h * vrab * (rhoab * r);
-      vis = (a.beta * miu - a.alpha * cab) * miu * (den * r);
+    R vis = 0;
+    if (vrab < R(0)) {
+      const R rhoab = R(0.5) * (t.rhoi + rhoj);
+      const R den = dist_sq + R(a.ata) * h * h;
+      const R r = rrcp<R>(den * rhoab);
+      const R miu = h * vrab * (rhoab * r);
+      vis = (R(a.beta) * miu - R(a.alpha) * cab) * miu * (den * r);
     }
-    if (a.lans) vis -= a.eps_half * dotvv * Vj;           // SPH_epsilon_mom
-    const double mpv = s.m[q] * (s.PV2[q] + t.pvsqi + vis);
-    const double g2 = -2 * ih * W;
-    double dw[3];
+    if (a.lans) vis -= R(a.eps_half) * dotvv * Vj;        // SPH_epsilon_mom
+    const R mpv = s.m[q] * (s.PV2[q] + t.pvsqi + vis);
+    const R g2 = R(-2) * ih * W;
+    R dw[3];
 #pragma unroll
     for (int k = 0; k < D; k++) dw[k] = g2 * si[k];
-    double heat = 0;
+    R heat = 0;
     if (a.lans && D == 3) {
       // k_ij F_ij of SPH_epsilon_heat_conductivity (6-arg, sph_lib.cc:1399-1439)
       // and compute_F (sph_lib.cc:257-281): 10 eps cp r^2 |v|^2 / (Pr sum|v_k dx_k|)
       // / (h c) * (dW.dx / r^2); the r^2 factors cancel.
-      double dotvr = 0, wr = 0;
+      R dotvr = 0, wr = 0;
 #pragma unroll
-      for (int k = 0; k < D; k++) { dotvr += fabs(vij[k] * dx[k]); wr += dw[k] * dx[k]; }
-      if (dotvr != 0 && wr != 0) {
-        const double cpab = 0.5 * (s.cp[q] + t.cpi);
-        const double kF = 10.0 * a.eps_half * cpab * dotvv * wr / (a.prandtl * dotvr * h * cab);
+      for (int k = 0; k < D; k++) { dotvr += rabs<R>(vij[k] * dx[k]); wr += dw[k] * dx[k]; }
+      if (dotvr != R(0) && wr != R(0)) {
+        const R cpab = R(0.5) * (s.cp[q] + t.cpi);
+        const R kF = R(10.0 * a.eps_half) * cpab * dotvv * wr * rrcp<R>(R(a.prandtl) * dotvr * h * cab);
         heat = s.m[q] * Vj * t.Vi * (t.Ti - s.T[q]) * kF;
       }
     }
-    double de = 0;
+    R de = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      const double f = mpv * dw[k];
-      t.rv[k] -= f;
+      const R f = mpv * dw[k];
+      t.rv[k] -= (double)f;
       de += f * vij[k];
     }
-    t.re += 0.5 * de + heat;
+    t.re += (double)(R(0.5) * de + heat);
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (t.bad) { raise_err(err, PSPH_EINVARIANT, i); return; }
     const double gz = -a.g, dt = a.dt;
     // v += dt rhs_v; gravity on the last axis; e += dt (rhs_e + v_last g)
-    // (mom_engr_update.cc:372-388)
+    // (mom_engr_update.cc:372-388); the velocity in the energy term is the
+    // FP64 one of the target
+    const double vl = a.dim == 3 ? (a.lans ? a.svz[i] : a.vz[i]) : (a.lans ? a.svx[i] : a.vx[i]);
     double nv0 = a.vx[i] + dt * (t.rv[0]);
     double nv1 = a.vy[i] + dt * (t.rv[1]);
     double nv2 = a.vz[i] + dt * (t.rv[2]);
-    double ne;
-    if (a.dim == 3) {
-      nv2 += dt * gz;
-      ne = a.e[i] + dt * (t.re + t.vel[2] * gz);
-    } else {
-      nv0 += dt * gz;
-      ne = a.e[i] + dt * (t.re + t.vel[0] * gz);
-    }
+    if (a.dim == 3) nv2 += dt * gz; else nv0 += dt * gz;
+    const double ne = a.e[i] + dt * (t.re + vl * gz);
     a.o_vx[i] = nv0; a.o_vy[i] = nv1; a.o_vz[i] = nv2;
     a.o_e[i] = ne;
     double p, c, T, gam, cp = a.cp[i];
@@ -486,67 +497,72 @@ struct RhsPass {
 // ---------------------------------------------------------------------------
 // Density pass: smooth_density (density_massfrac.cc:25-359)
 // ---------------------------------------------------------------------------
+struct DensArgs {
+  const double *h, *m, *rho, *e;
+  const int32_t *bc, *phase;
+  double *o_rho, *o_e, *xi, *p, *c, *T, *gam, *cp;
+  int32_t *inv;
+  double cutoff, cutoff2, e_ratio, msfrc_thresh;
+  int pghost, energy_smooth;
+  EosConst eos;
+  CutTest cut;
+};
+
+template <class R>
 struct DensPass {
-  struct Args {
-    const double *h, *m, *rho, *e;
-    const int32_t *bc, *phase;
-    double *o_rho, *o_e, *xi, *p, *c, *T, *gam, *cp;
-    int32_t *inv;
-    double cutoff, cutoff2, e_ratio, msfrc_thresh;
-    int pghost, energy_smooth;
-    EosConst eos;
-    CutTest cut;
-  };
+  using Args = DensArgs;
   struct Tile {
-    double h[kWave], m[kWave], rho[kWave], e[kWave];
+    double h[kWave];
+    R m[kWave], rho[kWave], e[kWave];
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
-  struct Tgt { double hi, hb, supp, r1, r2, wn, en; };
+  struct Tgt { double hi, supp, r1, r2, wn, en; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
-    t.hi = a.h[i]; t.hb = 0.5 * a.cutoff * kCutMargin; t.supp = a.cutoff2 * t.hi;
+    t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi;
     t.r1 = t.r2 = t.wn = t.en = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    s.h[l] = a.h[j]; s.m[l] = a.m[j]; s.rho[l] = a.rho[j]; s.e[l] = a.e[j];
+    s.h[l] = a.h[j]; s.m[l] = (R)a.m[j]; s.rho[l] = (R)a.rho[j]; s.e[l] = (R)a.e[j];
     int ph = a.phase[j];
     s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
   }
   // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
   // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
-  __device__ static double bound(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return (t.hi + s.h[q]) * t.hb; }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds) {
-    const double hij = 0.5 * (t.hi + s.h[q]);
-    const double ih = 1.0 / hij;
-    const double mj = s.m[q];
-    double expt = 0, amax = 0;
+    const double hijd = 0.5 * (t.hi + s.h[q]);
+    const R hij = (R)hijd;
+    const R ih = rrcp<R>(hij);
+    const R mj = s.m[q];
+    R expt = 0;
+    double amax = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      double sk = ds[k] * ih;
+      R sk = (R)ds[k] * ih;
       expt += sk * sk;
       amax = fmax(amax, fabs(ds[k]));
     }
-    const double nk = kInvSqrtPi * ih;
-    const double norm = (D == 3) ? nk * nk * nk : nk;
-    const double wm = norm * exp(-expt) * mj;
-    if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
+    const R nk = R(kInvSqrtPi) * ih;
+    const R norm = (D == 3) ? nk * nk * nk : nk;
+    const R wm = norm * rexp<R>(-expt) * mj;
+    if (s.ok[q] == 1) t.r1 += (double)wm; else t.r2 += (double)wm;
     if (a.energy_smooth) {
-      // weight(4 ds / h_ij, h_ij / 4) (HAVE_ENERGY_SMOOTH 1)
-      if (!a.cut.outside(a.e_ratio * amax, hij)) {
-        const double nke = kInvSqrtPi * (a.e_ratio * ih);
-        const double ne = (D == 3) ? nke * nke * nke : nke;
-        const double e2 = a.e_ratio * a.e_ratio;
-        const double wm_e = ne * exp(-(e2 * expt)) * mj;
-        const double vr = wm_e / s.rho[q];
-        t.wn += vr;
-        t.en += vr * s.e[q];
+      // weight(4 ds / h_ij, h_ij / 4) (HAVE_ENERGY_SMOOTH 1); exact FP64 box test
+      if (!a.cut.outside(a.e_ratio * amax, hijd)) {
+        const R nke = R(kInvSqrtPi) * (R(a.e_ratio) * ih);
+        const R ne = (D == 3) ? nke * nke * nke : nke;
+        const R e2 = R(a.e_ratio * a.e_ratio);
+        const R wm_e = ne * rexp<R>(-(e2 * expt)) * mj;
+        const R vr = wm_e / s.rho[q];
+        t.wn += (double)vr;
+        t.en += (double)(vr * s.e[q]);
       }
     }
   }
@@ -574,61 +590,66 @@ struct DensPass {
 // ---------------------------------------------------------------------------
 // LANS velocity filter: smooth_velocity (smooth_velocity.cc:25-165)
 // ---------------------------------------------------------------------------
+struct VsmArgs {
+  const double *h, *m, *rho, *vx, *vy, *vz;
+  const int32_t *bc;
+  double *svx, *svy, *svz;
+  double cutoff, cutoff2;
+};
+
+template <class R>
 struct VsmPass {
-  struct Args {
-    const double *h, *m, *rho, *vx, *vy, *vz;
-    const int32_t *bc;
-    double *svx, *svy, *svz;
-    double cutoff, cutoff2;
-  };
+  using Args = VsmArgs;
   struct Tile {
-    double mr[kWave], vx[kWave], vy[kWave], vz[kWave];
+    R mr[kWave], vx[kWave], vy[kWave], vz[kWave];
     int ok[kWave];
   };
-  struct Tgt { double hi, ih, bnd, supp, norm, v[3], dv[3], wn; };
+  struct Tgt { double hi, supp, v64[3], dv[3], wn; R ih, norm, v[3]; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
-    t.hi = a.h[i]; t.ih = 1.0 / t.hi; t.bnd = a.cutoff * t.hi * kCutMargin; t.supp = a.cutoff2 * t.hi;
-    const double nk = kInvSqrtPi * t.ih;
-    t.norm = a.vy ? nk * nk * nk : nk;
-    t.v[0] = a.vx[i]; t.v[1] = a.vy ? a.vy[i] : 0; t.v[2] = a.vz ? a.vz[i] : 0;
+    t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi;
+    const double ih = 1.0 / t.hi;
+    const double nk = kInvSqrtPi * ih;
+    t.ih = (R)ih;
+    t.norm = (R)(a.vy ? nk * nk * nk : nk);
+    t.v64[0] = a.vx[i]; t.v64[1] = a.vy ? a.vy[i] : 0; t.v64[2] = a.vz ? a.vz[i] : 0;
+    for (int k = 0; k < 3; k++) t.v[k] = (R)t.v64[k];
     t.dv[0] = t.dv[1] = t.dv[2] = 0; t.wn = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    s.mr[l] = a.m[j] / a.rho[j];
-    s.vx[l] = a.vx[j];
-    s.vy[l] = a.vy ? a.vy[j] : 0;
-    s.vz[l] = a.vz ? a.vz[j] : 0;
+    s.mr[l] = (R)(a.m[j] / a.rho[j]);
+    s.vx[l] = (R)a.vx[j];
+    s.vy[l] = a.vy ? (R)a.vy[j] : R(0);
+    s.vz[l] = a.vz ? (R)a.vz[j] : R(0);
     s.ok[l] = contr_vel(a.bc[j]);
   }
-  __device__ static double bound(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.bnd; }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds) {
-    double expt = 0;
+    R expt = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      double sk = ds[k] * t.ih;
+      R sk = (R)ds[k] * t.ih;
       expt += sk * sk;
     }
-    const double mvw = t.norm * exp(-expt) * s.mr[q];
-    t.dv[0] += mvw * (s.vx[q] - t.v[0]);
+    const R mvw = t.norm * rexp<R>(-expt) * s.mr[q];
+    t.dv[0] += (double)(mvw * (s.vx[q] - t.v[0]));
     if (D == 3) {
-      t.dv[1] += mvw * (s.vy[q] - t.v[1]);
-      t.dv[2] += mvw * (s.vz[q] - t.v[2]);
+      t.dv[1] += (double)(mvw * (s.vy[q] - t.v[1]));
+      t.dv[2] += (double)(mvw * (s.vz[q] - t.v[2]));
     }
-    t.wn += mvw;
+    t.wn += (double)mvw;
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
     const double iw = 1.0 / t.wn;
-    a.svx[i] = t.v[0] + t.dv[0] * iw;
-    if (a.svy) { a.svy[i] = t.v[1] + t.dv[1] * iw; a.svz[i] = t.v[2] + t.dv[2] * iw; }
+    a.svx[i] = t.v64[0] + t.dv[0] * iw;
+    if (a.svy) { a.svy[i] = t.v64[1] + t.dv[1] * iw; a.svz[i] = t.v64[2] + t.dv[2] * iw; }
   }
 };
 
@@ -637,28 +658,31 @@ struct VsmPass {
 // per iteration of the reference's loop; h is updated in place (no pass reads
 // another particle's h) and done[] marks particles whose |dh/h| < thresh.
 // ---------------------------------------------------------------------------
+struct AdaptArgs {
+  const double *m;
+  const int32_t *bc;
+  double *h;
+  uint32_t *done;
+  double cutoff, cutoff2, c_smooth, eta, ratio, thresh;
+  int pghost, dim;
+};
+
+template <class R>
 struct AdaptPass {
-  struct Args {
-    const double *m;
-    const int32_t *bc;
-    double *h;
-    uint32_t *done;
-    double cutoff, cutoff2, c_smooth, eta, ratio, thresh;
-    int pghost, dim;
-  };
+  using Args = AdaptArgs;
   struct Tile {
-    double m[kWave];
+    R m[kWave];
     int ok[kWave];
   };
-  struct Tgt { double mi, hcur, Hs, iHs, bnd, supp, norm, rho_star; bool done; };
+  struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; bool done; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &a, Tgt &t, int) {
     t.Hs = t.hcur * a.c_smooth;
-    t.iHs = 1.0 / t.Hs;
-    t.bnd = a.cutoff * t.Hs * kCutMargin;
+    const double iHs = 1.0 / t.Hs;
+    t.iHs = (R)iHs;
     t.supp = a.cutoff2 * t.Hs;
-    const double nk = kInvSqrtPi * t.iHs;
-    t.norm = a.dim == 3 ? nk * nk * nk : nk;
+    const double nk = kInvSqrtPi * iHs;
+    t.norm = (R)(a.dim == 3 ? nk * nk * nk : nk);
     t.rho_star = 0;
   }
   __device__ static bool want(const Tgt &t) { return !t.done; }
@@ -666,22 +690,21 @@ struct AdaptPass {
     t.mi = a.m[i]; t.hcur = a.h[i]; t.done = a.done[i] != 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    s.m[l] = a.m[j];
+    s.m[l] = (R)a.m[j];
     s.ok[l] = contr_dens(a.pghost, a.bc[j]);
   }
-  __device__ static double bound(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.bnd; }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.Hs; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *dx) {
-    double expt = 0;
+    R expt = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      double sk = dx[k] * t.iHs;
+      R sk = (R)dx[k] * t.iHs;
       expt += sk * sk;
     }
-    t.rho_star += t.norm * exp(-expt) * s.m[q];
+    t.rho_star += (double)(t.norm * rexp<R>(-expt) * s.m[q]);
   }
   __device__ static void end_pass(const Args &a, Tgt &t, int, uint32_t, bool active) {
     if (!active || t.done) return;
@@ -920,7 +943,7 @@ static int ensure_cub(psph_ctx *c, size_t bytes) {
 extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **out) {
   if (!cfg || !out) return fail(PSPH_EINVAL, "null argument");
   if (cfg->dim != 1 && cfg->dim != 3) return fail(PSPH_EINVAL, "dim must be 1 or 3");
-  if (cfg->precision != PSPH_FP64) return fail(PSPH_ENOTSUP, "only the FP64 form is built in this library version");
+  if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_PAIR) return fail(PSPH_EINVAL, "unknown precision form %d", cfg->precision);
   if (cfg->kernel != PSPH_KERNEL_GAUSS_BOX) return fail(PSPH_ENOTSUP, "only the Gaussian box kernel is built");
   if (!(cfg->h_air > 0)) return fail(PSPH_EINVAL, "h_air must be > 0");
   {
@@ -1255,6 +1278,13 @@ static void launch_pairs(psph_ctx *c, const PairCommon &pc, const typename P::Ar
     hipLaunchKernelGGL((k_pairs<1, Bound<1, P>>), nb, kWave, 0, c->stream, pc, a);
 }
 
+// FP64 form or FP32-pair form of a pass template
+template <template <class> class P>
+static void launch_form(psph_ctx *c, const PairCommon &pc, const typename P<double>::Args &a) {
+  if (c->cfg.precision == PSPH_FP32_PAIR) launch_pairs<P<float>>(c, pc, a);
+  else launch_pairs<P<double>>(c, pc, a);
+}
+
 static EosConst eos_const(const psph_config &k) {
   return EosConst{k.ng0, k.cvs, k.cvg, k.cva, k.rg, k.ra, k.lans};
 }
@@ -1277,7 +1307,7 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
   const double E = all ? 0.0 : 1.0 / c->grid.inv_e;
   // the list never reaches past the own bucket or the 4.25 h_i box of list time
   const double list_reach = std::max(k.list_factor * c->h_max, k.bucket_edge);
-  AdaptPass::Args a{};
+  AdaptArgs a{};
   a.m = c->f.m; a.bc = c->f.bc; a.h = c->f.h; a.done = c->usp;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.c_smooth = k.c_smooth; a.eta = k.eta;
   a.ratio = k.ratio_sml_dx; a.thresh = k.thresh; a.pghost = k.pghost_contrib; a.dim = k.dim;
@@ -1291,7 +1321,7 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
     }
     {
       StageTimer tm(c, 2);
-      if (c->n) launch_pairs<AdaptPass>(c, make_common(c, r), a);
+      if (c->n) launch_form<AdaptPass>(c, make_common(c, r), a);
     }
     HIP_TRY(hipGetLastError());
     PSPH_TRY(reduce_hmax(c, &hmax));
@@ -1313,7 +1343,7 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   double *src[9] = {c->f.vx, c->f.vy, c->f.vz, c->f.e, c->f.p, c->f.c, c->f.T, c->f.gam, c->f.cp};
   for (int q = 0; q < 9; q++)
     if (n) HIP_TRY(hipMemcpyAsync(ov[q], src[q], (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
-  RhsPass::Args a{};
+  RhsArgs a{};
   a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.cp = c->f.cp;
   a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.svx = c->f.svx; a.svy = c->f.svy; a.svz = c->f.svz;
   a.e = c->f.e; a.xi = c->f.xi; a.bc = c->f.bc;
@@ -1325,7 +1355,7 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 2, 0ull, 1);
   {
     StageTimer tm(c, 3);
-    if (n) launch_pairs<RhsPass>(c, make_common(c, c->r_main), a);
+    if (n) launch_form<RhsPass>(c, make_common(c, c->r_main), a);
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
@@ -1344,7 +1374,7 @@ extern "C" int psph_density(psph_ctx *c) {
     HIP_TRY(hipMemcpyAsync(c->dsp[1], c->f.rho, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->dsp[2], c->f.e, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
   }
-  DensPass::Args a{};
+  DensArgs a{};
   a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.e = c->f.e; a.bc = c->f.bc; a.phase = c->f.phase;
   a.o_rho = c->dsp[1]; a.o_e = c->dsp[2]; a.xi = c->f.xi; a.p = c->f.p; a.c = c->f.c; a.T = c->f.T;
   a.gam = c->f.gam; a.cp = c->f.cp; a.inv = c->f.inv;
@@ -1353,7 +1383,7 @@ extern "C" int psph_density(psph_ctx *c) {
   a.cut = cut_test(k.cutoff);
   {
     StageTimer tm(c, 4);
-    if (n) launch_pairs<DensPass>(c, make_common(c, c->r_main), a);
+    if (n) launch_form<DensPass>(c, make_common(c, c->r_main), a);
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.rho, c->dsp[1]);
@@ -1365,14 +1395,14 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   if (!c->grid_valid) return fail(PSPH_EINVAL, "vsmooth: call psph_neighbors first");
   const psph_config &k = c->cfg;
-  VsmPass::Args a{};
+  VsmArgs a{};
   a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.vx = c->f.vx;
   a.vy = k.dim == 3 ? c->f.vy : nullptr; a.vz = k.dim == 3 ? c->f.vz : nullptr;
   a.bc = c->f.bc; a.svx = c->f.svx; a.svy = k.dim == 3 ? c->f.svy : nullptr; a.svz = k.dim == 3 ? c->f.svz : nullptr;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2;
   {
     StageTimer tm(c, 5);
-    if (c->n) launch_pairs<VsmPass>(c, make_common(c, c->r_main), a);
+    if (c->n) launch_form<VsmPass>(c, make_common(c, c->r_main), a);
   }
   HIP_TRY(hipGetLastError());
   return check_err(c, "vsmooth");

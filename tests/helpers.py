@@ -87,3 +87,11 @@ def same_lists(off_a, idx_a, off_b, idx_b):
         if a.shape != b.shape or not np.array_equal(a, b):
             return i
     return -1
+
+
+def rel_l1(a, b):
+    """Relative L1 error sum|a-b| / sum|b| (the FP32-pair acceptance metric)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.sum(np.abs(b))
+    return float(np.sum(np.abs(a - b)) / den) if den > 0 else float(np.sum(np.abs(a)))
