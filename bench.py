@@ -106,6 +106,59 @@ def cpu_baseline(n_sample: float, steps: int, spacing: float):
                       f"{n_real} real particles, {steps} steps from t=0 ({wall:.1f} s)"}
 
 
+def run_form(cfg, P, args, form):
+    cap = len(P) + 400 * (args.warmup + args.steps) + 100000
+    G = psph.Solver(cfg, cap)
+    G.load(P)
+    t0 = time.perf_counter()
+    for s in range(args.warmup):
+        G.step()
+    sync()
+    log(f"[bench] {form}: {args.warmup} warm-up steps in {time.perf_counter() - t0:.1f}s")
+    _, n_real_start = G.count()
+    G.profile(True)
+    pairs = 0
+    sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        o = G.step()
+        pairs += o.n_pairs
+    sync()
+    wall = time.perf_counter() - t0
+    prof = G.profile_read()
+    n_total_end, n_real_end = G.count()
+    t_sim, step = G.time()
+    n_real = 0.5 * (n_real_start + n_real_end)
+    log(f"[bench] {form}: {args.steps} steps in {wall:.3f}s ({1e3 * wall / args.steps:.2f} ms/step), "
+        f"sim time {t_sim:.3f}s after step {step}")
+    for k, (ms, n) in prof.items():
+        log(f"[bench]   {k:11s} {ms:9.2f} ms over {n} launches")
+    final = None if args.no_compare else G.read()
+    G.close()
+    return {"value": n_real * args.steps / wall, "wall": wall, "prof": prof, "pairs": pairs,
+            "n_real": n_real, "n_total": n_total_end, "t_sim": t_sim, "step": step, "final": final,
+            "ms_per_step": 1e3 * wall / args.steps}
+
+
+def rel_l1_by_key(A, B):
+    """Per-field relative L1 of A against B over the particles present in both."""
+    def key(P):
+        k = P["key0"].astype(np.uint64) << np.uint64(32) | P["key1"].astype(np.uint64)
+        u, idx, cnt = np.unique(k, return_index=True, return_counts=True)
+        keep = idx[cnt == 1]                      # drop the (rare) shared Hilbert keys
+        return k[keep], keep
+    ka, sa = key(A)
+    kb, sb = key(B)
+    common, ia, ib = np.intersect1d(ka, kb, assume_unique=True, return_indices=True)
+    ia, ib = sa[ia], sb[ib]
+    out = {"particles_compared": int(len(common)), "particles_a": int(len(ka)), "particles_b": int(len(kb))}
+    for f in ("x", "y", "z", "vx", "vy", "vz", "rho", "e", "p", "c", "T", "xi", "h"):
+        a, b = A[f][ia], B[f][ib]
+        den = np.sum(np.abs(b))
+        out[f] = float(np.sum(np.abs(a - b)) / den) if den > 0 else 0.0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,7 +167,9 @@ def main():
                     help="untimed steps; >= 50 so the smoothing length has adapted once")
     ap.add_argument("--particles", type=float, default=4.0e6, help="real particles")
     ap.add_argument("--seed", type=int, default=20261204)
-    ap.add_argument("--precision", choices=["fp64", "fp32-pair"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp32-pair"], default="fp32-pair",
+                    help="form whose throughput is the reported value")
+    ap.add_argument("--no-compare", action="store_true", help="skip the run of the other form")
     ap.add_argument("--cpu-sample", type=float, default=6.0e4, help="real particles in the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -138,61 +193,55 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return
 
-    cfg = weak_plume_config(args.particles)
-    if args.precision == "fp32-pair":
-        cfg.precision = psph.PSPH_FP32_PAIR
+    base = weak_plume_config(args.particles)
     met = psph.meteo_standard()
     t0 = time.perf_counter()
-    P = psph.gen_plume(cfg, met, args.seed, 0.01)
+    P = psph.gen_plume(base, met, args.seed, 0.01)
     n_total0 = len(P)
     n_real0 = int(np.count_nonzero(P["bc"] == psph.BC_REAL))
-    log(f"[bench] generated {n_total0} particles ({n_real0} real, spacing {cfg.h_air:.2f} m) "
+    log(f"[bench] generated {n_total0} particles ({n_real0} real, spacing {base.h_air:.2f} m) "
         f"in {time.perf_counter() - t0:.1f}s")
-    cap = n_total0 + 400 * (args.warmup + args.steps) + 100000
-    G = psph.Solver(cfg, cap)
-    G.load(P)
-    del P
-    t0 = time.perf_counter()
-    for s in range(args.warmup):
-        G.step()
-    sync()
-    log(f"[bench] {args.warmup} warm-up steps in {time.perf_counter() - t0:.1f}s")
 
-    _, n_real_start = G.count()
-    G.profile(True)
-    pairs = 0
-    sync()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        o = G.step()
-        pairs += o.n_pairs
-    sync()
-    wall = time.perf_counter() - t0
-    prof = G.profile_read()
-    n_total_end, n_real_end = G.count()
-    t_sim, step = G.time()
-    n_real = 0.5 * (n_real_start + n_real_end)
-    value = n_real * args.steps / wall
-    log(f"[bench] {args.steps} steps in {wall:.3f}s ({1e3 * wall / args.steps:.2f} ms/step), "
-        f"sim time {t_sim:.3f}s after step {step}")
-    for k, (ms, n) in prof.items():
-        log(f"[bench]   {k:11s} {ms:9.2f} ms over {n} launches")
+    forms = [args.precision] + ([] if args.no_compare else [p for p in ("fp32-pair", "fp64") if p != args.precision])
+    runs = {}
+    for form in forms:
+        cfg = base.copy()
+        cfg.precision = psph.PSPH_FP32_PAIR if form == "fp32-pair" else psph.PSPH_FP64
+        runs[form] = run_form(cfg, P, args, form)
+    del P
+    main_run = runs[args.precision]
+    value, wall, prof, pairs, n_real = (main_run[k] for k in ("value", "wall", "prof", "pairs", "n_real"))
+    t_sim, step, n_total_end = main_run["t_sim"], main_run["step"], main_run["n_total"]
+    cfg = base
+
+    compare = None
+    if len(runs) == 2:
+        A, B = runs["fp32-pair"]["final"], runs["fp64"]["final"]
+        compare = {"fp64_value": runs["fp64"]["value"], "fp32_pair_value": runs["fp32-pair"]["value"],
+                   "speedup_fp32_pair_over_fp64": runs["fp32-pair"]["value"] / runs["fp64"]["value"],
+                   "rel_l1_fp32_pair_vs_fp64": rel_l1_by_key(A, B),
+                   "note": "both forms from the same input and step count; the FP64 form is the "
+                           "parity-checked one (1e-10 vs the CPU oracle on small cases)"}
+        log(f"[bench] fp32-pair / fp64 = {compare['speedup_fp32_pair_over_fp64']:.3f}; rel L1 "
+            + " ".join(f"{k}={v:.1e}" for k, v in compare["rel_l1_fp32_pair_vs_fp64"].items()))
+    for r in runs.values():
+        r.pop("final", None)
 
     rhs_ms, rhs_n = prof["rhs"]
     rhs_s = rhs_ms / max(rhs_n, 1) / 1e3
     bytes_per_launch = RHS_BYTES_PER_UPDATE * n_real
     achieved = bytes_per_launch / rhs_s / 1e9
-    traffic = traffic_from_profiles("rhs")
-    roofline = {"kernel": "k_pairs<RhsPass> (momentum/energy RHS)", "bound": "hbm",
+    traffic = traffic_from_profiles("rhs_" + args.precision)
+    kname = "k_pairs<3, Bound<3, RhsPass<%s>>> (momentum/energy RHS)" % ("float" if args.precision == "fp32-pair" else "double")
+    roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "avg_launch_ms": rhs_s * 1e3}
+                "avg_launch_ms": rhs_s * 1e3, "algorithmic_bytes_per_launch": bytes_per_launch}
     flops = RHS_FLOPS_PER_PAIR * (pairs / max(args.steps, 1)) / rhs_s / 1e12
     vpeak = FP64_VECTOR_PEAK_TFS if args.precision == "fp64" else FP32_VECTOR_PEAK_TFS
     roofline_valu = {"bound": "valu_fp64" if args.precision == "fp64" else "valu_fp32",
                      "achieved": flops, "peak": vpeak, "unit": "TFLOP/s", "frac": flops / vpeak,
                      "pairs_per_particle": pairs / max(args.steps, 1) / n_real}
-    G.close()
 
     cpu = None
     if not args.no_cpu:
@@ -214,6 +263,7 @@ def main():
                    "spacing_m": cfg.h_air, "vent": "r=54 m, 135 m/s, 1053 K, gas fraction 0.03",
                    "kernel": "gaussian-box", "steps_from": step - args.steps, "sim_time_s": t_sim},
         "roofline": roofline, "roofline_valu": roofline_valu, "cpu_baseline": cpu,
+        "forms": compare,
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
     }
     print(json.dumps(out))

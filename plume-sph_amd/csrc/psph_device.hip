@@ -46,6 +46,7 @@ using namespace psph;
 namespace {
 
 constexpr int kWave = 64;
+constexpr uint32_t kXcds = 8;      // MI355X: 8 XCDs, each with its own L2
 constexpr uint32_t kDead = 0xffffffffu;
 
 // ---------------------------------------------------------------------------
@@ -247,8 +248,15 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
   __shared__ uint32_t srefb[kWave];
   __shared__ typename P::Tile tile;
-  const uint32_t chunk = blockIdx.x;
-  if (chunk >= *pc.nchunks) return;
+  // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
+  // XCD x runs workgroups x, x+8, ...; give it the contiguous chunk range
+  // [x*per, (x+1)*per) so neighbouring chunks share that XCD's L2.
+  const uint32_t nch = *pc.nchunks;
+  const uint32_t per = (nch + kXcds - 1) / kXcds;
+  const uint32_t local = blockIdx.x / kXcds;
+  if (local >= per) return;
+  const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
+  if (chunk >= nch) return;
   const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk], row = pc.crow[chunk];
   const int lane = threadIdx.x;
   const bool active = (uint32_t)lane < cc;
@@ -1266,7 +1274,10 @@ static PairCommon make_common(psph_ctx *c, int r) {
 }
 
 static unsigned pair_grid(psph_ctx *c) {
-  return (unsigned)(c->n / kWave + (size_t)c->grid.ny * c->grid.nz + 1);
+  // upper bound on chunks (every row may add a partial chunk), padded so
+  // that kXcds * ceil(nchunks / kXcds) workgroups always exist
+  size_t ub = c->n / kWave + (size_t)c->grid.ny * c->grid.nz + 1 + kXcds;
+  return (unsigned)((ub + kXcds - 1) / kXcds * kXcds);
 }
 
 template <class P>
