@@ -140,6 +140,63 @@ def run_form(cfg, P, args, form):
             "ms_per_step": 1e3 * wall / args.steps}
 
 
+def adjoint_leg(args):
+    """Adjoint of the plume height: weak plume at the given size, N steps with a
+    checkpoint every K; backward time over the plain FP64 forward of the same
+    steps, and central differences (eps = 1e-6 relative, frozen schedule) on
+    the three vent parameters."""
+    sys.path.insert(0, os.path.join(ROOT, "plume-sph_amd"))
+    from psph.torch_adjoint import build_state
+    cfg = weak_plume_config(args.adjoint_particles)
+    met = psph.meteo_standard()
+    theta0 = np.array([cfg.vent_v, cfg.vent_T, 1.0 - cfg.ng0])
+    _, P = build_state(cfg, met, args.seed, 0.01, *theta0, None)
+    N, K = args.adjoint_steps, args.adjoint_ckpt
+    cap = len(P) + 400 * N + 50000
+    S = psph.Solver(cfg, cap)
+    S.load(P)
+    S.step()                              # first-touch / allocation warm-up
+    S.load(P)
+    sync()
+    t0 = time.perf_counter()
+    S.run(N)
+    sync()
+    fwd_ms = 1e3 * (time.perf_counter() - t0)
+    H_plain = S.diag_height()
+    S.load(P)
+    r = S.adjoint_run(N, K, grads=True)
+    log(f"[bench] adjoint: {len(P)} particles, {N} steps, ckpt {K}: forward {fwd_ms:.0f} ms, "
+        f"recorded forward {r['forward_ms']:.0f} ms, backward {r['backward_ms']:.0f} ms")
+    g = np.array([r["dH_dvent_v"], r["dH_dvent_T"], r["dH_dmass_frac"]])
+    fd = []
+    eps = 1e-6
+    for k in range(min(args.adjoint_fd, 3)):
+        d = np.zeros(3)
+        d[k] = 1.0
+        hs = []
+        for sgn in (1, -1):
+            c, Pp = build_state(cfg, met, args.seed, 0.01, *(theta0 * (1 + sgn * eps * d)), None)
+            S.set_vent(c.vent_v, c.vent_T, c.ng0)
+            S.load(Pp)
+            S.replay(N)
+            hs.append(S.diag_height())
+        fdv = (hs[0] - hs[1]) / (2 * eps)
+        ad = theta0[k] * g[k]
+        fd.append({"parameter": ["vent_v", "vent_T", "mass_frac"][k], "adjoint": ad, "fd": fdv,
+                   "rel_err": abs(ad - fdv) / abs(fdv) if fdv != 0 else None})
+        log(f"[bench] adjoint FD {fd[-1]['parameter']}: AD {ad:.9e} FD {fdv:.9e}")
+    S.close()
+    return {"workload": f"weak plume, {args.adjoint_particles / 1e3:g}k real particles, {N} steps, "
+                        f"checkpoint every {K}",
+            "particles_total": len(P), "H": r["H"], "H_plain_forward": H_plain,
+            "dH_dvent_v": r["dH_dvent_v"], "dH_dvent_T": r["dH_dvent_T"], "dH_dmass_frac": r["dH_dmass_frac"],
+            "forward_ms": fwd_ms, "recorded_forward_ms": r["forward_ms"], "backward_ms": r["backward_ms"],
+            "backward_over_forward": r["backward_ms"] / fwd_ms,
+            "fd_checks": fd, "fd_max_rel_err": max((x["rel_err"] or 0) for x in fd) if fd else None,
+            "note": "FD derivatives scaled by the parameter (d H / d ln theta); schedule (dt, h updates, "
+                    "inlet counts, ghost->real and escape events) frozen at the recorded run"}
+
+
 def rel_l1_by_key(A, B):
     """Per-field relative L1 of A against B over the particles present in both."""
     def key(P):
@@ -173,6 +230,11 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=6.0e4, help="real particles in the CPU sample")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--adjoint-particles", type=float, default=5.12e5)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-ckpt", type=int, default=20)
+    ap.add_argument("--adjoint-fd", type=int, default=3, help="vent parameters checked by central differences")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB: run only a plain device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -243,6 +305,14 @@ def main():
                      "achieved": flops, "peak": vpeak, "unit": "TFLOP/s", "frac": flops / vpeak,
                      "pairs_per_particle": pairs / max(args.steps, 1) / n_real}
 
+    adjoint = None
+    if not args.no_adjoint:
+        try:
+            adjoint = adjoint_leg(args)
+        except Exception as e:   # reported as missing, the forward figure stands
+            log(f"[bench] adjoint leg failed: {e}")
+            adjoint = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -264,6 +334,7 @@ def main():
                    "kernel": "gaussian-box", "steps_from": step - args.steps, "sim_time_s": t_sim},
         "roofline": roofline, "roofline_valu": roofline_valu, "cpu_baseline": cpu,
         "forms": compare,
+        "adjoint": adjoint,
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
     }
     print(json.dumps(out))

@@ -238,6 +238,45 @@ int psph_conservation(psph_ctx *ctx, double out[7]);
  * particles, H = sum m xi z / sum m xi.  No reference counterpart. */
 int psph_diag_height(psph_ctx *ctx, double *H);
 
+/* ---- adjoint ------------------------------------------------------------ */
+/* Reverse-mode gradient of the plume diagnostic H (psph_diag_height) after
+ * nsteps steps (psph_step) from the context's current state.  No reference
+ * counterpart: the reference has no adjoint.  Differentiated with respect to
+ * the vent exit velocity (cfg.vent_v), the vent temperature (cfg.vent_T), the
+ * gas mass fraction of the erupted material (cfg.ng0; its complement is the
+ * erupted solid mass fraction) and every particle's initial velocity, with
+ * the step's discrete schedule frozen at the recorded run: time steps,
+ * adapted smoothing lengths, inlet spawn counts, erupt-ghost -> real and
+ * escape events.  Erupt ghosts present at the start (bc 0) are taken to hold
+ * the vent state of the config, as psph_gen_plume makes them.
+ * The forward run stores a checkpoint every ckpt_every steps; the reverse
+ * sweep recomputes one segment at a time from its checkpoint.  grad_v and
+ * grad_sv (either may be NULL) receive dH/dv and dH/dsv of the starting
+ * state, 3 per particle, in psph_read order of that state (n_cap particles
+ * at most).  On return the context holds the state after nsteps steps, and
+ * the recorded schedule stays available to psph_replay_run. */
+typedef struct psph_adjoint_out {
+  double H;                  /* diagnostic at the end of the run */
+  double dH_dvent_v;         /* per m/s */
+  double dH_dvent_T;         /* per K */
+  double dH_dng0;            /* per unit gas mass fraction */
+  double dH_dmass_frac;      /* per unit erupted solid mass fraction (= -dH_dng0) */
+  double forward_ms;         /* recorded forward run incl. checkpoints */
+  double backward_ms;        /* recompute + reverse sweep */
+  int64_t steps, checkpoints, n_particles;
+  double reserved[4];
+} psph_adjoint_out;
+int psph_adjoint_run(psph_ctx *ctx, int64_t nsteps, int64_t ckpt_every, psph_adjoint_out *out,
+                     double *grad_v, double *grad_sv, size_t n_cap);
+/* Runs nsteps steps with the frozen schedule of the last psph_adjoint_run
+ * (the map the adjoint differentiates; finite-difference checks use it).
+ * The context must hold a state loaded at that run's start time and step,
+ * with the same particle count. */
+int psph_replay_run(psph_ctx *ctx, int64_t nsteps);
+/* Sets the vent exit velocity, temperature and gas mass fraction of the
+ * context's config and recomputes the derived vent state (psph_config_vent). */
+int psph_set_vent(psph_ctx *ctx, double vent_v, double vent_T, double ng0);
+
 /* Per-stage GPU time measured with HIP events on the context's stream.
  * Stage ids: 0 timestep, 1 neighbours (keys+sort+gather+ranges), 2 adapt_h,
  * 3 rhs, 4 density, 5 vsmooth, 6 update_pos, 7 inject.  psph_profile(ctx, 1)
@@ -249,8 +288,9 @@ int psph_profile_read(psph_ctx *ctx, double ms[PSPH_NSTAGES], int64_t launches[P
 
 const char *psph_last_error(void);
 const char *psph_version(void);
-/* sizeof(psph_config), sizeof(psph_soa), sizeof(psph_step_out) for FFI
- * bindings to check their struct layout: which = 0, 1, 2. */
+/* sizeof(psph_config), sizeof(psph_soa), sizeof(psph_step_out),
+ * sizeof(psph_adjoint_out) for FFI bindings to check their struct layout:
+ * which = 0, 1, 2, 3. */
 size_t psph_abi_sizeof(int which);
 
 #ifdef __cplusplus

@@ -1,0 +1,1178 @@
+// psph_adjoint.inl -- reverse-mode adjoint of the SPH step (part of psph_device.hip).
+//
+// Differentiates the plume diagnostic H = sum m xi z / sum m xi over real
+// particles after N steps with respect to the vent exit velocity, the vent
+// temperature, the gas mass fraction ng0 of the erupted material and every
+// particle's initial velocity.  The step map is differentiated with its
+// discrete schedule frozen: the time steps, the adapted smoothing lengths,
+// the inlet spawn counts and the erupt-ghost -> real and escape events of the
+// recorded run (Tape).  The finite-difference checks run the same frozen map
+// (psph_replay_run).
+//
+// Memory: a checkpoint of the whole state every K steps; a reverse segment
+// recomputes its K steps from the checkpoint, keeping each step's
+// intermediates (StepRec, ~256 B/particle/step), then sweeps them backwards.
+// The adjoint variables live in arrays indexed by persistent particle id, so
+// each step's cell-order permutation is a gather on entry and a scatter on
+// exit.  Pair terms are reversed by a gather over the same cell-row chunks as
+// the forward passes: a row particle collects both its own-role terms (it is
+// the particle whose sum holds the pair) and its neighbour-role terms (it is
+// the neighbour in a candidate's sum), so no atomics are needed and sums are
+// deterministic.  All adjoint arithmetic is FP64.
+
+namespace {
+
+enum LamField { L_X, L_Y, L_Z, L_VX, L_VY, L_VZ, L_SVX, L_SVY, L_SVZ, L_RHO, L_E, L_XI, L_P, L_C, L_T, L_CP, L_M, L_N };
+struct LamPtrs { double *f[L_N]; };
+
+__global__ void k_lam_gather(uint32_t n, const uint32_t *pid, LamPtrs src, LamPtrs dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t id = pid[i];
+#pragma unroll
+  for (int k = 0; k < L_N; k++) dst.f[k][i] = src.f[k][id];
+}
+__global__ void k_lam_scatter(uint32_t n, const uint32_t *pid, LamPtrs src, LamPtrs dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t id = pid[i];
+#pragma unroll
+  for (int k = 0; k < L_N; k++) dst.f[k][id] = src.f[k][i];
+}
+__global__ void k_fill_f64(double *p, double v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Seeds of H = sum m xi z / D, D = sum m xi (real particles).
+__global__ void k_lam_seed(uint32_t n, const int32_t *bc, const uint32_t *pid, const double *m,
+                           const double *xi, const double *z, double H, double D, LamPtrs L) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !is_real(bc[i])) return;
+  const uint32_t id = pid[i];
+  L.f[L_Z][id] = m[i] * xi[i] / D;
+  L.f[L_XI][id] = m[i] * (z[i] - H) / D;
+  L.f[L_M][id] = xi[i] * (z[i] - H) / D;
+}
+
+// update_pos reverse (update_pos.cc:34-168): x += dt sv (real) or dt v
+// (erupting ghost); x is carried, so only the velocity adjoint moves.
+__global__ void k_adj_pos(uint32_t n, const int32_t *bc, int erupting, int lans, double dt, LamPtrs l) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = bc[i];
+  int v0;
+  if (is_real(b)) v0 = lans ? L_SVX : L_VX;
+  else if (b == 0 && erupting) v0 = L_VX;
+  else return;
+#pragma unroll
+  for (int k = 0; k < 3; k++) l.f[v0 + k][i] += dt * l.f[L_X + k][i];
+}
+
+// Velocity filter reverse, per target: sv'' = sum_j w_ij v'_j / Wn, so with
+// a = lambda_sv'' / Wn the pair weight gets a.v'_j - a.sv''.  sv is
+// overwritten for targets: its adjoint restarts at zero.
+__global__ void k_adj_vsm_pre(uint32_t n, const int32_t *bc, const double *wn, const double *s0,
+                              const double *s1, const double *s2, LamPtrs l, double *ax, double *ay,
+                              double *az, double *ac) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!need_neigh(bc[i])) { ax[i] = ay[i] = az[i] = ac[i] = 0; return; }
+  const double iw = 1.0 / wn[i];
+  const double a0 = l.f[L_SVX][i] * iw, a1 = l.f[L_SVY][i] * iw, a2 = l.f[L_SVZ][i] * iw;
+  ax[i] = a0; ay[i] = a1; az[i] = a2;
+  ac[i] = a0 * s0[i] + a1 * s1[i] + a2 * s2[i];
+  l.f[L_SVX][i] = l.f[L_SVY][i] = l.f[L_SVZ][i] = 0;
+}
+
+// d/d(rho, e, xi, ng0) of lp p + lc c + lT T + lcp Cp for the closed-form EOS
+// (particle.cc:528-604): p = rho e u, c^2 = (u + u^2) e / G, T = e / Cvm,
+// Cp = Cvm (LANS), u = Rm / Cvm, G = ng + na.
+__device__ inline void eos_adj(const EosConst &k, double rho, double e, double xi, double lp, double lc,
+                               double lT, double lcp, double &grho, double &ge, double &gxi, double &gng0) {
+  const double ng0 = k.ng0;
+  const double na = 1 - xi, ng = ng0 * xi, ns = xi - ng;
+  const double Cvm = ns * k.cvs + ng * k.cvg + na * k.cva;
+  const double Rm = ng * k.rg + na * k.ra;
+  const double G = ng + na;
+  const double u = Rm / Cvm;
+  const double c = sqrt((u + u * u) * e / G);
+  const double T = e / Cvm;
+  double gu = lp * rho * e;
+  double gCvm = -lT * T / Cvm + (k.lans ? lcp : 0.0);
+  double gG = 0;
+  ge = lp * rho * u + lT / Cvm;
+  grho = lp * e * u;
+  if (c > 0) {
+    gu += lc * c * (1 + 2 * u) / (2 * (u + u * u));
+    ge += lc * c / (2 * e);
+    gG -= lc * c / (2 * G);
+  }
+  gCvm -= gu * u / Cvm;
+  const double gRm = gu / Cvm;
+  gxi = gCvm * ((1 - ng0) * k.cvs + ng0 * k.cvg - k.cva) + gRm * (ng0 * k.rg - k.ra) - gG * (1 - ng0);
+  gng0 = gCvm * xi * (k.cvg - k.cvs) + gRm * xi * k.rg + gG * xi;
+}
+
+// Density reverse, per target: EOS, then rho = r1 + r2, xi = r2 / rho,
+// e = En / Wn (energy smoothing).  The pass overwrites rho, xi, e and the EOS
+// outputs of targets, so their adjoints restart at zero (e keeps the direct
+// term when energy smoothing is off).
+__global__ void k_adj_dens_pre(uint32_t n, const int32_t *bc, const double *rho2, const double *e2,
+                               const double *xi2, const double *wne, EosConst eos, int energy_smooth,
+                               LamPtrs l, double *s1, double *s2, double *sen, double *swn, double *gng0) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!need_neigh(bc[i])) { s1[i] = s2[i] = sen[i] = swn[i] = 0; gng0[i] = 0; return; }
+  const double rho = rho2[i], e = e2[i], xi = xi2[i];
+  double grho, ge, gxi, gn;
+  eos_adj(eos, rho, e, xi, l.f[L_P][i], l.f[L_C][i], l.f[L_T][i], l.f[L_CP][i], grho, ge, gxi, gn);
+  grho += l.f[L_RHO][i];
+  ge += l.f[L_E][i];
+  gxi += l.f[L_XI][i];
+  gng0[i] = gn;
+  s1[i] = grho - gxi * xi / rho;
+  s2[i] = grho + gxi * (1 - xi) / rho;
+  if (energy_smooth) {
+    const double iw = 1.0 / wne[i];
+    sen[i] = ge * iw;
+    swn[i] = -ge * e * iw;
+    l.f[L_E][i] = 0;
+  } else {
+    sen[i] = swn[i] = 0;
+    l.f[L_E][i] = ge;
+  }
+  l.f[L_RHO][i] = 0; l.f[L_XI][i] = 0; l.f[L_P][i] = 0; l.f[L_C][i] = 0; l.f[L_T][i] = 0;
+  if (eos.lans) l.f[L_CP][i] = 0;
+}
+
+// RHS reverse, per target: v' = v + dt (rhs_v + g), e' = e + dt (rhs_e + vl g):
+// the pair seeds are dt lambda_v', dt lambda_e'; v and e carry lambda through.
+__global__ void k_adj_rhs_pre(uint32_t n, const int32_t *bc, double dt, double gz, int lans, LamPtrs l,
+                              double *a0, double *a1, double *a2, double *ae) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!need_neigh(bc[i])) { a0[i] = a1[i] = a2[i] = ae[i] = 0; return; }
+  const double le = l.f[L_E][i];
+  a0[i] = dt * l.f[L_VX][i]; a1[i] = dt * l.f[L_VY][i]; a2[i] = dt * l.f[L_VZ][i];
+  ae[i] = dt * le;
+  l.f[lans ? L_SVZ : L_VZ][i] += dt * gz * le;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint pair skeleton.  Rows: the particles whose state can depend on the
+// inputs (real particles, erupt ghosts, boundary ghosts next to targets).  For row i and
+// candidate j it evaluates the forward pass's exact pair predicate twice:
+// with i as the summing particle (own role, i a target) and with j as the
+// summing particle (neighbour role, j a target).  dx passed to own() is
+// x_i - x_j, to nbr() x_j - x_i: always summing minus neighbour.
+// ---------------------------------------------------------------------------
+struct AdjCommon {
+  const double *x, *y, *z, *h, *h_list;
+  const uint32_t *refb, *cellkey, *cs, *rows, *cfirst, *ccount, *crow, *nchunks;
+  const int32_t *bc;
+  int nx, ny, nz, r, list_all;
+  double list_factor, cutoff2;
+  CutTest cut;
+};
+
+template <class P>
+__global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::Args a) {
+  __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], slf[kWave];
+  __shared__ uint32_t srefb[kWave];
+  __shared__ int sbc[kWave];
+  __shared__ typename P::Tile tile;
+  const uint32_t nch = *pc.nchunks;
+  const uint32_t per = (nch + kXcds - 1) / kXcds;
+  const uint32_t local = blockIdx.x / kXcds;
+  if (local >= per) return;
+  const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
+  if (chunk >= nch) return;
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk], row = pc.crow[chunk];
+  const int lane = threadIdx.x;
+  const bool active = (uint32_t)lane < cc;
+  const uint32_t i = pc.rows[cf + (active ? lane : 0)];
+  const int cx0 = (int)(pc.cellkey[pc.rows[cf]] % (uint32_t)pc.nx);
+  const int cx1 = (int)(pc.cellkey[pc.rows[cf + cc - 1]] % (uint32_t)pc.nx);
+  const int cy = (int)(row % (uint32_t)pc.ny), cz = (int)(row / (uint32_t)pc.ny);
+
+  const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  const uint32_t rb_i = pc.refb[i];
+  const double hi = pc.h[i];
+  const double lf_i = pc.list_factor * pc.h_list[i];
+  const double supp_i = pc.cutoff2 * hi;
+  const bool tgt_i = need_neigh(pc.bc[i]);
+  typename P::Row t;
+  P::load_row(a, i, t);
+
+  const int xlo = max(cx0 - pc.r, 0), xhi = min(cx1 + pc.r, pc.nx - 1);
+  for (int dz = -pc.r; dz <= pc.r; dz++) {
+    const int zz = cz + dz;
+    if (zz < 0 || zz >= pc.nz) continue;
+    for (int dy = -pc.r; dy <= pc.r; dy++) {
+      const int yy = cy + dy;
+      if (yy < 0 || yy >= pc.ny) continue;
+      const size_t rbase = ((size_t)zz * pc.ny + yy) * pc.nx;
+      const uint32_t lo = pc.cs[rbase + xlo], hi_ = pc.cs[rbase + xhi + 1];
+      for (uint32_t base = lo; base < hi_; base += kWave) {
+        const uint32_t j = base + lane;
+        if (j < hi_) {
+          sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+          sh[lane] = pc.h[j];
+          slf[lane] = pc.list_factor * pc.h_list[j];
+          srefb[lane] = pc.refb[j];
+          sbc[lane] = pc.bc[j];
+          P::stage(a, j, tile, lane);
+        }
+        __syncthreads();
+        const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
+        if (active) {
+          uint64_t mo = 0, mn = 0;
+          for (uint32_t q = 0; q < cnt; q++) {
+            const double d0 = xi[0] - sx[q], d1 = xi[1] - sy[q], d2 = xi[2] - sz[q];
+            const double amax = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
+            const bool same = pc.list_all || srefb[q] == rb_i;
+            const uint32_t jq = base + q;
+            if (tgt_i && P::own_ok(t, tile, q, jq, i) && (same || amax <= lf_i) && amax <= supp_i &&
+                !pc.cut.outside(amax, P::hker_own(hi, sh[q])))
+              mo |= 1ull << q;
+            if (need_neigh(sbc[q]) && P::nbr_ok(t, tile, q, jq, i) && (same || amax <= slf[q]) &&
+                amax <= pc.cutoff2 * sh[q] && !pc.cut.outside(amax, P::hker_nbr(hi, sh[q])))
+              mn |= 1ull << q;
+          }
+          while (mo) {
+            const uint32_t q = (uint32_t)__builtin_ctzll(mo);
+            mo &= mo - 1;
+            const double dx[3] = {xi[0] - sx[q], xi[1] - sy[q], xi[2] - sz[q]};
+            const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+            P::own(a, t, tile, q, dx, amax, hi, sh[q]);
+          }
+          while (mn) {
+            const uint32_t q = (uint32_t)__builtin_ctzll(mn);
+            mn &= mn - 1;
+            const double dx[3] = {sx[q] - xi[0], sy[q] - xi[1], sz[q] - xi[2]};
+            const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+            P::nbr(a, t, tile, q, dx, amax, hi, sh[q]);
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (active) P::finish(a, t, i);
+}
+
+__device__ inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// --- velocity filter (smooth_velocity.cc:25-165) ----------------------------
+struct VsmAdj {
+  struct Args {
+    const double *m, *rho, *v0, *v1, *v2;       // rho'' and v' of the step
+    const double *ax, *ay, *az, *ac;             // per-target seeds
+    const int32_t *bc;
+    LamPtrs l;
+  };
+  struct Tile { double mr[kWave], v[3][kWave], a[3][kWave], c[kWave]; int ok[kWave]; };
+  struct Row { double mi, rhoi, mr, v[3], a[3], c, g[3], gv[3], gm, grho; bool ok; };
+  __device__ static void load_row(const Args &A, uint32_t i, Row &t) {
+    t.mi = A.m[i]; t.rhoi = A.rho[i]; t.mr = t.mi / t.rhoi;
+    t.v[0] = A.v0[i]; t.v[1] = A.v1[i]; t.v[2] = A.v2[i];
+    t.a[0] = A.ax[i]; t.a[1] = A.ay[i]; t.a[2] = A.az[i]; t.c = A.ac[i];
+    for (int k = 0; k < 3; k++) t.g[k] = t.gv[k] = 0;
+    t.gm = t.grho = 0;
+    t.ok = contr_vel(A.bc[i]);
+  }
+  __device__ static void stage(const Args &A, uint32_t j, Tile &s, int l) {
+    s.mr[l] = A.m[j] / A.rho[j];
+    s.v[0][l] = A.v0[j]; s.v[1][l] = A.v1[j]; s.v[2][l] = A.v2[j];
+    s.a[0][l] = A.ax[j]; s.a[1][l] = A.ay[j]; s.a[2][l] = A.az[j]; s.c[l] = A.ac[j];
+    s.ok[l] = contr_vel(A.bc[j]);
+  }
+  __device__ static double hker_own(double hi, double) { return hi; }
+  __device__ static double hker_nbr(double, double hj) { return hj; }
+  __device__ static bool own_ok(const Row &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
+  __device__ static bool nbr_ok(const Row &t, const Tile &, uint32_t, uint32_t, uint32_t) { return t.ok; }
+  __device__ static void own(const Args &, Row &t, const Tile &s, uint32_t q, const double *dx, double,
+                             double hs, double) {
+    const double ih2 = 1.0 / (hs * hs);
+    const double nk = kInvSqrtPi / hs;
+    const double w = nk * nk * nk * exp(-dot3(dx, dx) * ih2) * s.mr[q];
+    const double gw = t.a[0] * s.v[0][q] + t.a[1] * s.v[1][q] + t.a[2] * s.v[2][q] - t.c;
+    const double f = -2.0 * gw * w * ih2;
+    for (int k = 0; k < 3; k++) t.g[k] += f * dx[k];
+  }
+  __device__ static void nbr(const Args &, Row &t, const Tile &s, uint32_t q, const double *dx, double,
+                             double, double hs) {
+    const double ih2 = 1.0 / (hs * hs);
+    const double nk = kInvSqrtPi / hs;
+    const double w = nk * nk * nk * exp(-dot3(dx, dx) * ih2) * t.mr;
+    const double a[3] = {s.a[0][q], s.a[1][q], s.a[2][q]};
+    const double gw = dot3(a, t.v) - s.c[q];
+    for (int k = 0; k < 3; k++) t.gv[k] += a[k] * w;
+    t.gm += gw * w / t.mi;
+    t.grho -= gw * w / t.rhoi;
+    const double f = -2.0 * gw * w * ih2;
+    for (int k = 0; k < 3; k++) t.g[k] -= f * dx[k];
+  }
+  __device__ static void finish(const Args &A, Row &t, uint32_t i) {
+    for (int k = 0; k < 3; k++) { A.l.f[L_X + k][i] += t.g[k]; A.l.f[L_VX + k][i] += t.gv[k]; }
+    A.l.f[L_M][i] += t.gm;
+    A.l.f[L_RHO][i] += t.grho;
+  }
+};
+
+// --- density + energy smoothing (density_massfrac.cc:25-359) ----------------
+struct DensAdj {
+  struct Args {
+    const double *m, *rho, *e1;                  // rho of the step's start (energy weights), e'
+    const int32_t *bc, *phase;
+    const double *s1, *s2, *sen, *swn;           // per-target seeds
+    double e_ratio;
+    int pghost, energy_smooth;
+    CutTest cut;
+    LamPtrs l;
+  };
+  struct Tile { double m[kWave], rho[kWave], e1[kWave], s1[kWave], s2[kWave], sen[kWave], swn[kWave]; int ok[kWave]; };
+  struct Row { double mi, rhoi, e1i, s1, s2, sen, swn, g[3], gm, grho, ge; int ok; };
+  __device__ static void load_row(const Args &A, uint32_t i, Row &t) {
+    t.mi = A.m[i]; t.rhoi = A.rho[i]; t.e1i = A.e1[i];
+    t.s1 = A.s1[i]; t.s2 = A.s2[i]; t.sen = A.sen[i]; t.swn = A.swn[i];
+    const int ph = A.phase[i];
+    t.ok = (contr_dens(A.pghost, A.bc[i]) && (ph == 1 || ph == 2)) ? ph : 0;
+    t.g[0] = t.g[1] = t.g[2] = 0;
+    t.gm = t.grho = t.ge = 0;
+  }
+  __device__ static void stage(const Args &A, uint32_t j, Tile &s, int l) {
+    s.m[l] = A.m[j]; s.rho[l] = A.rho[j]; s.e1[l] = A.e1[j];
+    s.s1[l] = A.s1[j]; s.s2[l] = A.s2[j]; s.sen[l] = A.sen[j]; s.swn[l] = A.swn[j];
+    const int ph = A.phase[j];
+    s.ok[l] = (contr_dens(A.pghost, A.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
+  }
+  __device__ static double hker_own(double hi, double hj) { return 0.5 * (hi + hj); }
+  __device__ static double hker_nbr(double hi, double hj) { return 0.5 * (hj + hi); }
+  __device__ static bool own_ok(const Row &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
+  __device__ static bool nbr_ok(const Row &t, const Tile &, uint32_t, uint32_t, uint32_t) { return t.ok != 0; }
+  // dF/d(r^2) of the pair, and the energy weight terms
+  __device__ static void own(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
+                             double hi, double hj) {
+    const double h = 0.5 * (hi + hj);
+    const double ih2 = 1.0 / (h * h);
+    const double r2 = dot3(dx, dx);
+    const double nk = kInvSqrtPi / h;
+    const double wm = nk * nk * nk * exp(-r2 * ih2) * s.m[q];
+    const double lr = s.ok[q] == 1 ? t.s1 : t.s2;
+    double f = lr * wm * ih2;
+    if (A.energy_smooth && !A.cut.outside(A.e_ratio * amax, h)) {
+      const double e2 = A.e_ratio * A.e_ratio;
+      const double nke = kInvSqrtPi * A.e_ratio / h;
+      const double vr = nke * nke * nke * exp(-e2 * r2 * ih2) * s.m[q] / s.rho[q];
+      const double gvr = t.swn + t.sen * s.e1[q];
+      f += gvr * vr * e2 * ih2;
+    }
+    for (int k = 0; k < 3; k++) t.g[k] -= 2.0 * f * dx[k];
+  }
+  __device__ static void nbr(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
+                             double hi, double hj) {
+    const double h = 0.5 * (hj + hi);
+    const double ih2 = 1.0 / (h * h);
+    const double r2 = dot3(dx, dx);
+    const double nk = kInvSqrtPi / h;
+    const double wm = nk * nk * nk * exp(-r2 * ih2) * t.mi;
+    const double lr = t.ok == 1 ? s.s1[q] : s.s2[q];
+    double f = lr * wm * ih2;
+    t.gm += lr * wm / t.mi;
+    if (A.energy_smooth && !A.cut.outside(A.e_ratio * amax, h)) {
+      const double e2 = A.e_ratio * A.e_ratio;
+      const double nke = kInvSqrtPi * A.e_ratio / h;
+      const double vr = nke * nke * nke * exp(-e2 * r2 * ih2) * t.mi / t.rhoi;
+      const double gvr = s.swn[q] + s.sen[q] * t.e1i;
+      t.ge += s.sen[q] * vr;
+      t.gm += gvr * vr / t.mi;
+      t.grho -= gvr * vr / t.rhoi;
+      f += gvr * vr * e2 * ih2;
+    }
+    for (int k = 0; k < 3; k++) t.g[k] += 2.0 * f * dx[k];
+  }
+  __device__ static void finish(const Args &A, Row &t, uint32_t i) {
+    for (int k = 0; k < 3; k++) A.l.f[L_X + k][i] += t.g[k];
+    A.l.f[L_M][i] += t.gm;
+    A.l.f[L_RHO][i] += t.grho;
+    A.l.f[L_E][i] += t.ge;
+  }
+};
+
+// --- momentum / energy RHS (mom_engr_update.cc:30-439) ----------------------
+struct RhsAdjConst { double alpha, beta, ata, eps_half, prandtl; int lans; };
+struct RhsSide { double m, rho, p, c, T, cp, vel[3]; };
+struct RhsGrad { double dx[3], vij[3], m_n, rho_s, rho_n, p_s, p_n, c_s, c_n, T_s, T_n, cp_s, cp_n; };
+
+// Reverse of one pair term of the summing particle s with neighbour n,
+// dx = x_s - x_n, seeds av = dt lambda_v'_s, ae = dt lambda_e'_s:
+//   L = av . (-M dW) + ae (M dW.v_sn / 2 + heat),  dW = G dx,
+//   G = -2 W / h^2, M = m_n (p_n / rho_n^2 + p_s / rho_s^2 + Pi).
+__device__ inline void rhs_pair_adj(const RhsAdjConst &k, const double *dx, double h, const RhsSide &S,
+                                    const RhsSide &N, const double *av, double ae, RhsGrad &g) {
+  const double ih2 = 1.0 / (h * h);
+  const double r2 = dot3(dx, dx);
+  const double nk = kInvSqrtPi / h;
+  const double W = nk * nk * nk * exp(-r2 * ih2);
+  const double G = -2.0 * ih2 * W;
+  const double vij[3] = {S.vel[0] - N.vel[0], S.vel[1] - N.vel[1], S.vel[2] - N.vel[2]};
+  const double vrab = dot3(dx, vij), dotvv = dot3(vij, vij);
+  const double Vn = 1.0 / N.rho, Vs = 1.0 / S.rho;
+  const double cab = 0.5 * (S.c + N.c);
+  const bool avis = vrab < 0;
+  double vis_a = 0, mu = 0, rhoab = 1, den = 1;
+  if (avis) {
+    rhoab = 0.5 * (S.rho + N.rho);
+    den = r2 + k.ata * h * h;
+    mu = h * vrab / den;
+    vis_a = (k.beta * mu - k.alpha * cab) * mu / rhoab;
+  }
+  double vis = vis_a;
+  if (k.lans) vis -= k.eps_half * dotvv * Vn;
+  const double PV2n = N.p * Vn * Vn, PV2s = S.p * Vs * Vs;
+  const double Msum = PV2n + PV2s + vis;
+  const double mpv = N.m * Msum;
+  const double Q = -dot3(av, dx) + 0.5 * ae * vrab;
+  const double g_mpv = G * Q, g_Q = mpv * G;
+  double g_G = mpv * Q;
+  double g_r2 = 0, g_dotvv = 0, g_cab = 0, g_vrab = 0.5 * ae * g_Q;
+  double g_dx[3], g_vij[3] = {0, 0, 0};
+  for (int q = 0; q < 3; q++) g_dx[q] = -g_Q * av[q];
+  g.m_n = g_mpv * Msum;
+  const double g_vis = g_mpv * N.m;
+  g.p_n = g_vis * Vn * Vn;
+  g.rho_n = -2.0 * g_vis * PV2n * Vn;
+  g.p_s = g_vis * Vs * Vs;
+  g.rho_s = -2.0 * g_vis * PV2s * Vs;
+  if (k.lans) {
+    g_dotvv -= g_vis * k.eps_half * Vn;
+    g.rho_n += g_vis * k.eps_half * dotvv * Vn * Vn;
+  }
+  if (avis) {
+    const double g_mu = g_vis * (2.0 * k.beta * mu - k.alpha * cab) / rhoab;
+    g_cab -= g_vis * k.alpha * mu / rhoab;
+    const double g_rhoab = -g_vis * vis_a / rhoab;
+    g.rho_s += 0.5 * g_rhoab;
+    g.rho_n += 0.5 * g_rhoab;
+    g_vrab += g_mu * h / den;
+    g_r2 -= g_mu * mu / den;
+  }
+  g.T_s = g.T_n = g.cp_s = g.cp_n = 0;
+  if (k.lans) {
+    // heat: m_n V_n V_s (T_s - T_n) 10 eps cp r^2 |v|^2 / (Pr sum|v_k dx_k| h c) (dW.dx / r^2)
+    double dotvr = 0;
+    for (int q = 0; q < 3; q++) dotvr += fabs(vij[q] * dx[q]);
+    const double Gd = G * r2;
+    if (dotvr != 0 && Gd != 0) {
+      const double cpab = 0.5 * (S.cp + N.cp);
+      const double B = N.m * Vn * Vs * (10.0 * k.eps_half / k.prandtl) * cpab * dotvv * Gd / (dotvr * h * cab);
+      const double heat = B * (S.T - N.T);
+      g.T_s = ae * B;
+      g.T_n = -ae * B;
+      const double hh = ae * heat;
+      g.m_n += hh / N.m;
+      g.rho_n -= hh * Vn;
+      g.rho_s -= hh * Vs;
+      g.cp_s = g.cp_n = 0.5 * hh / cpab;
+      g_dotvv += hh / dotvv;
+      const double g_Gd = hh / Gd;
+      g_G += g_Gd * r2;
+      g_r2 += g_Gd * G;
+      const double g_dotvr = -hh / dotvr;
+      for (int q = 0; q < 3; q++) {
+        const double pr = vij[q] * dx[q];
+        const double sg = pr > 0 ? 1.0 : (pr < 0 ? -1.0 : 0.0);
+        g_vij[q] += g_dotvr * sg * dx[q];
+        g_dx[q] += g_dotvr * sg * vij[q];
+      }
+      g_cab -= hh / cab;
+    }
+  }
+  g_r2 -= g_G * G * ih2;
+  for (int q = 0; q < 3; q++) {
+    g.dx[q] = g_dx[q] + 2.0 * dx[q] * g_r2 + g_vrab * vij[q];
+    g.vij[q] = g_vij[q] + g_vrab * dx[q] + 2.0 * vij[q] * g_dotvv;
+  }
+  g.c_s = g.c_n = 0.5 * g_cab;
+}
+
+struct RhsAdj {
+  struct Args {
+    const double *m, *rho, *p, *c, *T, *cp, *u0, *u1, *u2;   // u: sv (LANS) or v
+    const double *a0, *a1, *a2, *ae;                          // per-target seeds
+    RhsAdjConst k;
+    LamPtrs l;
+  };
+  struct Tile { double m[kWave], rho[kWave], p[kWave], c[kWave], T[kWave], cp[kWave], u[3][kWave], a[3][kWave], ae[kWave]; };
+  struct Row { RhsSide me; double a[3], ae, g[3], gu[3], gm, grho, gp, gc, gT, gcp; };
+  __device__ static void load_row(const Args &A, uint32_t i, Row &t) {
+    t.me = RhsSide{A.m[i], A.rho[i], A.p[i], A.c[i], A.T[i], A.cp[i], {A.u0[i], A.u1[i], A.u2[i]}};
+    t.a[0] = A.a0[i]; t.a[1] = A.a1[i]; t.a[2] = A.a2[i]; t.ae = A.ae[i];
+    for (int k = 0; k < 3; k++) t.g[k] = t.gu[k] = 0;
+    t.gm = t.grho = t.gp = t.gc = t.gT = t.gcp = 0;
+  }
+  __device__ static void stage(const Args &A, uint32_t j, Tile &s, int l) {
+    s.m[l] = A.m[j]; s.rho[l] = A.rho[j]; s.p[l] = A.p[j]; s.c[l] = A.c[j]; s.T[l] = A.T[j]; s.cp[l] = A.cp[j];
+    s.u[0][l] = A.u0[j]; s.u[1][l] = A.u1[j]; s.u[2][l] = A.u2[j];
+    s.a[0][l] = A.a0[j]; s.a[1][l] = A.a1[j]; s.a[2][l] = A.a2[j]; s.ae[l] = A.ae[j];
+  }
+  __device__ static RhsSide side(const Tile &s, uint32_t q) {
+    return RhsSide{s.m[q], s.rho[q], s.p[q], s.c[q], s.T[q], s.cp[q], {s.u[0][q], s.u[1][q], s.u[2][q]}};
+  }
+  __device__ static double hker_own(double hi, double hj) { return 0.5 * (hi + hj); }
+  __device__ static double hker_nbr(double hi, double hj) { return 0.5 * (hj + hi); }
+  __device__ static bool own_ok(const Row &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
+  __device__ static bool nbr_ok(const Row &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
+  __device__ static void own(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double,
+                             double hi, double hj) {
+    RhsGrad g;
+    rhs_pair_adj(A.k, dx, 0.5 * (hi + hj), t.me, side(s, q), t.a, t.ae, g);
+    for (int k = 0; k < 3; k++) { t.g[k] += g.dx[k]; t.gu[k] += g.vij[k]; }
+    t.grho += g.rho_s; t.gp += g.p_s; t.gc += g.c_s; t.gT += g.T_s; t.gcp += g.cp_s;
+  }
+  __device__ static void nbr(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double,
+                             double hi, double hj) {
+    RhsGrad g;
+    const double a[3] = {s.a[0][q], s.a[1][q], s.a[2][q]};
+    rhs_pair_adj(A.k, dx, 0.5 * (hj + hi), side(s, q), t.me, a, s.ae[q], g);
+    for (int k = 0; k < 3; k++) { t.g[k] -= g.dx[k]; t.gu[k] -= g.vij[k]; }
+    t.gm += g.m_n; t.grho += g.rho_n; t.gp += g.p_n; t.gc += g.c_n; t.gT += g.T_n; t.gcp += g.cp_n;
+  }
+  __device__ static void finish(const Args &A, Row &t, uint32_t i) {
+    const int u0 = A.k.lans ? L_SVX : L_VX;
+    for (int k = 0; k < 3; k++) { A.l.f[L_X + k][i] += t.g[k]; A.l.f[u0 + k][i] += t.gu[k]; }
+    A.l.f[L_M][i] += t.gm; A.l.f[L_RHO][i] += t.grho; A.l.f[L_P][i] += t.gp;
+    A.l.f[L_C][i] += t.gc; A.l.f[L_T][i] += t.gT; A.l.f[L_CP][i] += t.gcp;
+  }
+};
+
+// cells holding at least one target (the benign race writes the same value)
+__global__ void k_cell_has_target(uint32_t n, const int32_t *bc, const uint32_t *ckey, uint32_t *tcell) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && need_neigh(bc[i])) tcell[ckey[i]] = 1u;
+}
+// Adjoint rows: real particles, erupt ghosts, and boundary ghosts (bc 1, 2:
+// constant in time, but their velocity and state enter their neighbours'
+// sums) whose cell neighbourhood holds a target.
+__global__ void k_adj_row_flags(uint32_t n, const int32_t *bc, const uint32_t *ckey, const uint32_t *tcell,
+                                int nx, int ny, int nz, uint32_t *flags) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int b = i < n ? bc[i] : -1;
+  uint32_t f = (is_real(b) || b == 0) ? 1u : 0u;
+  if (b == 1 || b == 2) {
+    const uint32_t k = ckey[i];
+    const int cx = (int)(k % (uint32_t)nx), cy = (int)((k / (uint32_t)nx) % (uint32_t)ny), cz = (int)(k / ((uint32_t)nx * ny));
+    for (int dz = -1; dz <= 1 && !f; dz++)
+      for (int dy = -1; dy <= 1 && !f; dy++)
+        for (int dx = -1; dx <= 1 && !f; dx++) {
+          const int x = cx + dx, y = cy + dy, z = cz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
+          if (tcell[((size_t)z * ny + y) * nx + x]) f = 1u;
+        }
+  }
+  flags[i] = f;
+}
+
+// fixed-order block sums of one array
+__global__ void k_sum_blocks(uint32_t n, const double *v, double *part) {
+  __shared__ double sh[256];
+  double acc = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += v[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+}  // namespace
+
+// ===========================================================================
+// Checkpoints, step records and the adjoint context
+// ===========================================================================
+#define PSPH_NDF 19
+#define PSPH_NUF 3
+#define PSPH_NIF 3
+
+struct Snapshot {
+  size_t n = 0, cap = 0;
+  double *d[PSPH_NDF] = {};
+  uint32_t *u[PSPH_NUF] = {};
+  int32_t *i[PSPH_NIF] = {};
+  uint32_t *pid = nullptr;
+  double time = 0, next_output = 0;
+  int64_t step = 0, tk = 0;
+  int ioutput = 0;
+  uint32_t next_pid = 0;
+  unsigned long long dead = 0;
+};
+
+struct Adjoint {
+  std::vector<Snapshot> ck;
+  Snapshot fin;
+  std::vector<StepRec> recs;
+  size_t lam_cap = 0, l_cap = 0;
+  double *L[L_N] = {};        // by particle id
+  double *l[L_N] = {};        // cell order of the step being reversed
+  double *aux[5] = {};        // per-particle seeds of one pass
+  double *part = nullptr;     // reduction partials
+  // chunks of adjoint rows
+  uint32_t *flags = nullptr, *tpos = nullptr, *rows = nullptr, *row_chunks = nullptr, *row_off = nullptr;
+  uint32_t *cfirst = nullptr, *ccount = nullptr, *crow = nullptr, *nchunks = nullptr;
+  uint32_t *tcell = nullptr;
+  size_t n_cap = 0, rows_cap = 0, chunk_cap = 0, tcell_cap = 0;
+};
+
+static void dev_fields(psph_ctx *c, double **d, uint32_t **u, int32_t **i) {
+  int k = 0;
+#define D_(a) d[k++] = c->f.a;
+  PSPH_DFIELDS(D_)
+#undef D_
+  k = 0;
+#define U_(a) u[k++] = c->f.a;
+  PSPH_UFIELDS(U_)
+#undef U_
+  k = 0;
+#define I_(a) i[k++] = c->f.a;
+  PSPH_IFIELDS(I_)
+#undef I_
+}
+
+static void snap_free(Snapshot &s) {
+  for (auto &p : s.d) { if (p) hipFree(p); p = nullptr; }
+  for (auto &p : s.u) { if (p) hipFree(p); p = nullptr; }
+  for (auto &p : s.i) { if (p) hipFree(p); p = nullptr; }
+  if (s.pid) hipFree(s.pid);
+  s.pid = nullptr;
+  s.cap = s.n = 0;
+}
+
+static void rec_free(StepRec &r) {
+  for (auto &p : r.d) { if (p) hipFree(p); p = nullptr; }
+  void *ptrs[] = {r.pid, r.refb, r.ckey, r.bc, r.phase, r.cs};
+  for (auto p : ptrs) if (p) hipFree(p);
+  r.pid = r.refb = r.ckey = nullptr;
+  r.bc = r.phase = nullptr;
+  r.cs = nullptr;
+  r.cap = r.cs_cap = r.n = 0;
+}
+
+static void adjoint_free(psph_ctx *c) {
+  Adjoint *A = c->adj;
+  if (!A) return;
+  for (auto &s : A->ck) snap_free(s);
+  snap_free(A->fin);
+  for (auto &r : A->recs) rec_free(r);
+  for (auto p : A->L) if (p) hipFree(p);
+  for (auto p : A->l) if (p) hipFree(p);
+  for (auto p : A->aux) if (p) hipFree(p);
+  void *ptrs[] = {A->part, A->flags, A->tpos, A->rows, A->row_chunks, A->row_off, A->cfirst, A->ccount, A->crow,
+                  A->nchunks, A->tcell};
+  for (auto p : ptrs) if (p) hipFree(p);
+  delete A;
+  c->adj = nullptr;
+}
+
+static int snap_save(psph_ctx *c, Snapshot &s) {
+  const size_t n = c->n;
+  if (n > s.cap) {
+    snap_free(s);
+    const size_t cap = std::max<size_t>(n, 64);
+    for (auto &p : s.d) PSPH_TRY(dmalloc((void **)&p, cap * 8));
+    for (auto &p : s.u) PSPH_TRY(dmalloc((void **)&p, cap * 4));
+    for (auto &p : s.i) PSPH_TRY(dmalloc((void **)&p, cap * 4));
+    PSPH_TRY(dmalloc((void **)&s.pid, cap * 4));
+    s.cap = cap;
+  }
+  double *d[PSPH_NDF]; uint32_t *u[PSPH_NUF]; int32_t *iv[PSPH_NIF];
+  dev_fields(c, d, u, iv);
+  hipStream_t st = c->stream;
+  if (n) {
+    for (int k = 0; k < PSPH_NDF; k++) HIP_TRY(hipMemcpyAsync(s.d[k], d[k], n * 8, hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < PSPH_NUF; k++) HIP_TRY(hipMemcpyAsync(s.u[k], u[k], n * 4, hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < PSPH_NIF; k++) HIP_TRY(hipMemcpyAsync(s.i[k], iv[k], n * 4, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(s.pid, c->pid, n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(&s.dead, c->d_u64 + 3, 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  s.n = n;
+  s.time = c->time; s.next_output = c->next_output; s.step = c->step; s.ioutput = c->ioutput;
+  s.next_pid = c->next_pid; s.tk = c->tape.k;
+  return 0;
+}
+
+static int snap_load(psph_ctx *c, const Snapshot &s) {
+  double *d[PSPH_NDF]; uint32_t *u[PSPH_NUF]; int32_t *iv[PSPH_NIF];
+  dev_fields(c, d, u, iv);
+  hipStream_t st = c->stream;
+  const size_t n = s.n;
+  if (n) {
+    for (int k = 0; k < PSPH_NDF; k++) HIP_TRY(hipMemcpyAsync(d[k], s.d[k], n * 8, hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < PSPH_NUF; k++) HIP_TRY(hipMemcpyAsync(u[k], s.u[k], n * 4, hipMemcpyDeviceToDevice, st));
+    for (int k = 0; k < PSPH_NIF; k++) HIP_TRY(hipMemcpyAsync(iv[k], s.i[k], n * 4, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->pid, s.pid, n * 4, hipMemcpyDeviceToDevice, st));
+  }
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, st, c->d_u64 + 3, (unsigned long long)s.dead, 1);
+  HIP_TRY(hipStreamSynchronize(st));
+  c->n = n;
+  c->time = s.time; c->next_output = s.next_output; c->step = s.step; c->ioutput = s.ioutput;
+  c->next_pid = s.next_pid; c->tape.k = s.tk;
+  c->grid_valid = false;
+  return 0;
+}
+
+// stage 0: state after sort and h update; 1: after RHS; 2: after density;
+// 3: after the velocity filter
+static int rec_capture(psph_ctx *c, int stage, double dt) {
+  StepRec &r = *c->rec;
+  hipStream_t st = c->stream;
+  const size_t n = c->n;
+  auto cp = [&](int f, const double *src) -> int {
+    if (n) HIP_TRY(hipMemcpyAsync(r.d[f], src, n * 8, hipMemcpyDeviceToDevice, st));
+    return 0;
+  };
+  if (stage == 0) {
+    if (n > r.cap) {
+      const size_t cap = std::max(c->cap, n);
+      for (auto &p : r.d) { if (p) hipFree(p); p = nullptr; }
+      uint32_t **u32s[] = {&r.pid, &r.refb, &r.ckey};
+      for (auto pp : u32s) { if (*pp) hipFree(*pp); *pp = nullptr; }
+      if (r.bc) hipFree(r.bc);
+      if (r.phase) hipFree(r.phase);
+      r.bc = r.phase = nullptr;
+      for (auto &p : r.d) PSPH_TRY(dmalloc((void **)&p, cap * 8));
+      for (auto pp : u32s) PSPH_TRY(dmalloc((void **)pp, cap * 4));
+      PSPH_TRY(dmalloc((void **)&r.bc, cap * 4));
+      PSPH_TRY(dmalloc((void **)&r.phase, cap * 4));
+      r.cap = cap;
+    }
+    const size_t ncs = (size_t)c->grid.ncells + 1;
+    if (ncs > r.cs_cap) {
+      if (r.cs) hipFree(r.cs);
+      r.cs = nullptr;
+      PSPH_TRY(dmalloc((void **)&r.cs, ncs * 4));
+      r.cs_cap = ncs;
+    }
+    r.n = n;
+    r.grid = c->grid;
+    r.r_main = c->r_main;
+    r.dt = dt;
+    r.time = c->time;
+    const double *src[] = {c->f.x, c->f.y, c->f.z, c->f.h, c->f.m, c->f.rho, c->f.p, c->f.c, c->f.T, c->f.cp,
+                           c->f.svx, c->f.svy, c->f.svz, c->f.vx, c->f.vy, c->f.vz, c->f.e, c->h_list};
+    for (int f = RF_X; f <= RF_HLIST; f++) PSPH_TRY(cp(f, src[f]));
+    if (n) {
+      HIP_TRY(hipMemcpyAsync(r.pid, c->pid, n * 4, hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(r.refb, c->refb, n * 4, hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(r.ckey, c->key_a, n * 4, hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(r.bc, c->f.bc, n * 4, hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(r.phase, c->f.phase, n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    HIP_TRY(hipMemcpyAsync(r.cs, c->cs, ncs * 4, hipMemcpyDeviceToDevice, st));
+  } else if (stage == 1) {
+    PSPH_TRY(cp(RF_V1X, c->f.vx)); PSPH_TRY(cp(RF_V1Y, c->f.vy)); PSPH_TRY(cp(RF_V1Z, c->f.vz));
+    PSPH_TRY(cp(RF_E1, c->f.e));
+  } else if (stage == 2) {
+    PSPH_TRY(cp(RF_RHO2, c->f.rho)); PSPH_TRY(cp(RF_E2, c->f.e)); PSPH_TRY(cp(RF_XI2, c->f.xi));
+  } else {
+    PSPH_TRY(cp(RF_SV2X, c->f.svx)); PSPH_TRY(cp(RF_SV2Y, c->f.svy)); PSPH_TRY(cp(RF_SV2Z, c->f.svz));
+    r.erupting = iferupt(c);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+static int adj_arrays(psph_ctx *c, Adjoint &A, size_t nid) {
+  if (nid > A.lam_cap) {
+    for (auto &p : A.L) { if (p) hipFree(p); p = nullptr; }
+    const size_t cap = std::max<size_t>(nid, 64);
+    for (auto &p : A.L) PSPH_TRY(dmalloc((void **)&p, cap * 8));
+    A.lam_cap = cap;
+  }
+  if (c->cap > A.l_cap) {
+    for (auto &p : A.l) { if (p) hipFree(p); p = nullptr; }
+    for (auto &p : A.aux) { if (p) hipFree(p); p = nullptr; }
+    for (auto &p : A.l) PSPH_TRY(dmalloc((void **)&p, c->cap * 8));
+    for (auto &p : A.aux) PSPH_TRY(dmalloc((void **)&p, c->cap * 8));
+    A.l_cap = c->cap;
+  }
+  if (!A.part) PSPH_TRY(dmalloc((void **)&A.part, kRedBlocks * 8));
+  return 0;
+}
+
+static LamPtrs lam_ptrs(double *const *p) {
+  LamPtrs r;
+  for (int k = 0; k < L_N; k++) r.f[k] = p[k];
+  return r;
+}
+
+// deterministic sum of v[0..n)
+static int dev_sum(psph_ctx *c, Adjoint &A, const double *v, size_t n, double *out) {
+  *out = 0;
+  if (!n) return 0;
+  hipLaunchKernelGGL(k_sum_blocks, kRedBlocks, 256, 0, c->stream, (uint32_t)n, v, A.part);
+  std::vector<double> part(kRedBlocks);
+  HIP_TRY(hipMemcpyAsync(part.data(), A.part, kRedBlocks * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double s = 0;
+  for (double x : part) s += x;
+  *out = s;
+  return 0;
+}
+
+// chunks of adjoint rows (real particles and erupt ghosts) of one step
+static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
+  const uint32_t n = (uint32_t)R.n;
+  const GridDesc &g = R.grid;
+  const uint32_t nrows = (uint32_t)g.ny * g.nz;
+  hipStream_t s = c->stream;
+  if (n + 1 > A.n_cap) {
+    void *ptrs[] = {A.flags, A.tpos, A.rows};
+    for (auto p : ptrs) if (p) hipFree(p);
+    const size_t cap = std::max<size_t>(c->cap, n) + 1;
+    PSPH_TRY(dmalloc((void **)&A.flags, cap * 4));
+    PSPH_TRY(dmalloc((void **)&A.tpos, cap * 4));
+    PSPH_TRY(dmalloc((void **)&A.rows, cap * 4));
+    A.n_cap = cap;
+  }
+  if (nrows + 1 > A.rows_cap) {
+    if (A.row_chunks) hipFree(A.row_chunks);
+    if (A.row_off) hipFree(A.row_off);
+    PSPH_TRY(dmalloc((void **)&A.row_chunks, (nrows + 1) * 4));
+    PSPH_TRY(dmalloc((void **)&A.row_off, (nrows + 1) * 4));
+    A.rows_cap = nrows + 1;
+  }
+  const size_t chunks_ub = n / kWave + nrows + 1;
+  if (chunks_ub > A.chunk_cap) {
+    void *ptrs[] = {A.cfirst, A.ccount, A.crow};
+    for (auto p : ptrs) if (p) hipFree(p);
+    PSPH_TRY(dmalloc((void **)&A.cfirst, chunks_ub * 4));
+    PSPH_TRY(dmalloc((void **)&A.ccount, chunks_ub * 4));
+    PSPH_TRY(dmalloc((void **)&A.crow, chunks_ub * 4));
+    A.chunk_cap = chunks_ub;
+  }
+  if (!A.nchunks) PSPH_TRY(dmalloc((void **)&A.nchunks, 4));
+  const size_t ncells = (size_t)g.nx * g.ny * g.nz;
+  if (ncells > A.tcell_cap) {
+    if (A.tcell) hipFree(A.tcell);
+    A.tcell = nullptr;
+    PSPH_TRY(dmalloc((void **)&A.tcell, ncells * 4));
+    A.tcell_cap = ncells;
+  }
+  hipLaunchKernelGGL(k_fill_i32, blocks(ncells), 256, 0, s, (int *)A.tcell, 0, (int)ncells);
+  if (n) hipLaunchKernelGGL(k_cell_has_target, blocks(n), 256, 0, s, n, R.bc, R.ckey, A.tcell);
+  hipLaunchKernelGGL(k_adj_row_flags, blocks(n + 1), 256, 0, s, n, R.bc, R.ckey, A.tcell, g.nx, g.ny, g.nz, A.flags);
+  size_t bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, A.flags, A.tpos, (int)n + 1, s));
+  PSPH_TRY(ensure_cub(c, bytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, A.flags, A.tpos, (int)n + 1, s));
+  if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, A.flags, A.tpos, A.rows);
+  hipLaunchKernelGGL(k_row_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, R.cs, A.tpos, A.row_chunks);
+  bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, A.row_chunks, A.row_off, (int)nrows + 1, s));
+  PSPH_TRY(ensure_cub(c, bytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, A.row_chunks, A.row_off, (int)nrows + 1, s));
+  hipLaunchKernelGGL(k_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, R.cs, A.tpos, A.row_off,
+                     A.cfirst, A.ccount, A.crow, A.nchunks);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <class P>
+static void launch_adj(psph_ctx *c, const AdjCommon &pc, const StepRec &R, const typename P::Args &a) {
+  size_t ub = R.n / kWave + (size_t)R.grid.ny * R.grid.nz + 1 + kXcds;
+  unsigned nb = (unsigned)((ub + kXcds - 1) / kXcds * kXcds);
+  hipLaunchKernelGGL(k_adj_pairs<P>, nb, kWave, 0, c->stream, pc, a);
+}
+
+struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };
+
+// vent state derivatives (parameters.h:958-975): rho_v = p / (Rg T) / ng0,
+// e_v = Cvm(1) T, m_v = rho_v V / np, T = e_v / Cvm(1) = T_v, Cp = Cvm(1)
+struct VentDeriv { double drho_dT, drho_dn, de_dT, de_dn, dm_dT, dm_dn, dcp_dn; };
+static VentDeriv vent_deriv(const psph_config &k) {
+  VentDeriv d;
+  d.drho_dT = -k.vent_rho / k.vent_T;
+  d.drho_dn = -k.vent_rho / k.ng0;
+  d.de_dT = k.ng0 * k.cvg + (1 - k.ng0) * k.cvs;
+  d.de_dn = (k.cvg - k.cvs) * k.vent_T;
+  d.dm_dT = -k.vent_mass / k.vent_T;
+  d.dm_dn = -k.vent_mass / k.ng0;
+  d.dcp_dn = k.cvg - k.cvs;
+  return d;
+}
+
+// theta-gradient of erupt ghosts created with column velocity vel (z of
+// injected ghosts moves with vel: dz/dvel = injection time; setup ghosts sit
+// at fixed z).  Mass terms are added once at the end.
+static void ghost_theta(const psph_config &k, const VentDeriv &d, double vel, double tz, const double *lam,
+                        ThetaGrad &g) {
+  // lam: z, vz, rho, e, T, cp
+  const double dvel = k.vent_v != 0 ? vel / k.vent_v : 0.0;
+  g.v += lam[1] * dvel + lam[0] * tz * dvel;
+  g.T += lam[2] * d.drho_dT + lam[3] * d.de_dT + lam[4];
+  g.ng0 += lam[2] * d.drho_dn + lam[3] * d.de_dn + lam[5] * d.dcp_dn;
+}
+
+static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
+  const psph_config &k = c->cfg;
+  hipStream_t st = c->stream;
+  const uint32_t n = (uint32_t)R.n;
+  const VentDeriv vd = vent_deriv(k);
+  // 1. inlet: the ghosts this step injected
+  if (R.inj_hi > R.inj_lo) {
+    const size_t m = R.inj_hi - R.inj_lo;
+    const int fl[6] = {L_Z, L_VZ, L_RHO, L_E, L_T, L_CP};
+    std::vector<double> lam(6 * m);
+    for (int f = 0; f < 6; f++)
+      HIP_TRY(hipMemcpyAsync(lam.data() + f * m, A.L[fl[f]] + R.inj_lo, m * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (R.inj_vel.size() != m) return fail(PSPH_EINVARIANT, "adjoint: inlet record of %zu ghosts, %zu ids", R.inj_vel.size(), m);
+    for (size_t q = 0; q < m; q++) {
+      double one[6];
+      for (int f = 0; f < 6; f++) one[f] = lam[f * m + q];
+      ghost_theta(k, vd, R.inj_vel[q], R.time, one, th);
+    }
+  }
+  if (!n) return 0;
+  const LamPtrs Lid = lam_ptrs(A.L), l = lam_ptrs(A.l);
+  hipLaunchKernelGGL(k_lam_gather, blocks(n), 256, 0, st, n, R.pid, Lid, l);
+  // 2. position update
+  hipLaunchKernelGGL(k_adj_pos, blocks(n), 256, 0, st, n, R.bc, R.erupting, k.lans, R.dt, l);
+  PSPH_TRY(adj_rows(c, A, R));
+  AdjCommon pc;
+  pc.x = R.d[RF_X]; pc.y = R.d[RF_Y]; pc.z = R.d[RF_Z]; pc.h = R.d[RF_H]; pc.h_list = R.d[RF_HLIST];
+  pc.refb = R.refb; pc.cellkey = R.ckey; pc.cs = R.cs; pc.rows = A.rows;
+  pc.cfirst = A.cfirst; pc.ccount = A.ccount; pc.crow = A.crow; pc.nchunks = A.nchunks;
+  pc.bc = R.bc;
+  pc.nx = R.grid.nx; pc.ny = R.grid.ny; pc.nz = R.grid.nz; pc.r = R.r_main;
+  pc.list_all = k.list_all ? 1 : 0;
+  pc.list_factor = k.list_factor; pc.cutoff2 = k.cutoff2;
+  pc.cut = cut_test(k.cutoff);
+  // 3. velocity filter
+  if (k.lans) {
+    hipLaunchKernelGGL(k_adj_vsm_pre, blocks(n), 256, 0, st, n, R.bc, R.d[RF_WNV], R.d[RF_SV2X], R.d[RF_SV2Y],
+                       R.d[RF_SV2Z], l, A.aux[0], A.aux[1], A.aux[2], A.aux[3]);
+    VsmAdj::Args a{R.d[RF_M], R.d[RF_RHO2], R.d[RF_V1X], R.d[RF_V1Y], R.d[RF_V1Z], A.aux[0], A.aux[1], A.aux[2], A.aux[3], R.bc, l};
+    launch_adj<VsmAdj>(c, pc, R, a);
+  }
+  // 4. density + EOS
+  {
+    hipLaunchKernelGGL(k_adj_dens_pre, blocks(n), 256, 0, st, n, R.bc, R.d[RF_RHO2], R.d[RF_E2], R.d[RF_XI2],
+                       R.d[RF_WNE], eos_const(k), k.energy_smooth, l, A.aux[0], A.aux[1], A.aux[2], A.aux[3], A.aux[4]);
+    double gn = 0;
+    PSPH_TRY(dev_sum(c, A, A.aux[4], n, &gn));
+    th.ng0 += gn;
+    DensAdj::Args a{R.d[RF_M], R.d[RF_RHO], R.d[RF_E1], R.bc, R.phase, A.aux[0], A.aux[1], A.aux[2], A.aux[3],
+                    k.e_smooth_ratio, k.pghost_contrib, k.energy_smooth, cut_test(k.cutoff), l};
+    launch_adj<DensAdj>(c, pc, R, a);
+  }
+  // 5. momentum / energy
+  {
+    hipLaunchKernelGGL(k_adj_rhs_pre, blocks(n), 256, 0, st, n, R.bc, R.dt, -k.g, k.lans, l,
+                       A.aux[0], A.aux[1], A.aux[2], A.aux[3]);
+    RhsAdj::Args a;
+    a.m = R.d[RF_M]; a.rho = R.d[RF_RHO]; a.p = R.d[RF_P]; a.c = R.d[RF_C]; a.T = R.d[RF_T]; a.cp = R.d[RF_CP];
+    a.u0 = k.lans ? R.d[RF_SVX] : R.d[RF_VX];
+    a.u1 = k.lans ? R.d[RF_SVY] : R.d[RF_VY];
+    a.u2 = k.lans ? R.d[RF_SVZ] : R.d[RF_VZ];
+    a.a0 = A.aux[0]; a.a1 = A.aux[1]; a.a2 = A.aux[2]; a.ae = A.aux[3];
+    a.k = RhsAdjConst{k.alpha, k.beta, k.ata, k.eps_half, k.prandtl, k.lans};
+    a.l = l;
+    launch_adj<RhsAdj>(c, pc, R, a);
+  }
+  hipLaunchKernelGGL(k_lam_scatter, blocks(n), 256, 0, st, n, R.pid, l, Lid);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every, psph_adjoint_out *out,
+                                double *grad_v, double *grad_sv, size_t n_cap) {
+  if (!c || !out) return fail(PSPH_EINVAL, "null argument");
+  const psph_config &k = c->cfg;
+  if (k.dim != 3) return fail(PSPH_ENOTSUP, "adjoint: built for the 3D plume only");
+  if (nsteps < 1 || ckpt_every < 1) return fail(PSPH_EINVAL, "adjoint: nsteps and ckpt_every must be >= 1");
+  if (c->tape.mode) return fail(PSPH_EINVAL, "adjoint: a recorded run is already in progress");
+  if (!c->adj) c->adj = new Adjoint();
+  Adjoint &A = *c->adj;
+  std::memset(out, 0, sizeof *out);
+  hipStream_t st = c->stream;
+  HIP_TRY(hipStreamSynchronize(st));
+  const double t_start = now_ms();
+
+  // ---- forward, recording the schedule ----
+  Tape &tp = c->tape;
+  tape_clear(tp);
+  const size_t n0 = c->n;
+  if (n0) hipLaunchKernelGGL(k_iota_u32, blocks(n0), 256, 0, st, c->pid, 0u, (uint32_t)n0);
+  c->next_pid = (uint32_t)n0;
+  PSPH_TRY(tape_ids(c, std::max<size_t>(n0, 1)));
+  hipLaunchKernelGGL(k_fill_i32, blocks(tp.id_cap), 256, 0, st, tp.tr_step, -1, (int)tp.id_cap);
+  hipLaunchKernelGGL(k_fill_i32, blocks(tp.id_cap), 256, 0, st, tp.esc_step, -1, (int)tp.id_cap);
+  tp.t0 = c->time; tp.step0 = c->step; tp.n0 = n0;
+  tp.mode = 1;
+  const int64_t K = std::min(ckpt_every, nsteps);
+  const int64_t nseg = (nsteps + K - 1) / K;
+  if ((int64_t)A.ck.size() < nseg) A.ck.resize(nseg);
+  if ((int64_t)A.recs.size() < K) A.recs.resize(K);
+  const int64_t last0 = (nseg - 1) * K;
+  int rc = 0;
+  for (int64_t s = 0; s < nsteps && !rc; s++) {
+    if (s % K == 0) rc = snap_save(c, A.ck[s / K]);
+    if (rc) break;
+    c->rec = s >= last0 ? &A.recs[s - last0] : nullptr;
+    rc = psph_step(c, nullptr);
+  }
+  c->rec = nullptr;
+  if (rc) { tp.mode = 0; return rc; }
+  double H = 0;
+  PSPH_TRY(psph_diag_height(c, &H));
+  PSPH_TRY(snap_save(c, A.fin));
+  HIP_TRY(hipStreamSynchronize(st));
+  const double t_fwd = now_ms();
+
+  // ---- seeds ----
+  PSPH_TRY(adj_arrays(c, A, c->next_pid));
+  for (int f = 0; f < L_N; f++)
+    hipLaunchKernelGGL(k_fill_f64, blocks(A.lam_cap), 256, 0, st, A.L[f], 0.0, A.lam_cap);
+  {
+    double cons[8];
+    uint32_t n = (uint32_t)c->n;
+    hipLaunchKernelGGL(k_conservation, kRedBlocks, 256, 0, st, n, c->f.bc, c->f.m, c->f.e, c->f.vx,
+                       c->f.vy, c->f.vz, c->f.xi, c->f.z, c->d_part);
+    std::vector<double> part(kRedBlocks * 8);
+    HIP_TRY(hipMemcpyAsync(part.data(), c->d_part, part.size() * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double D = 0;
+    for (int b = 0; b < kRedBlocks; b++) D += part[b * 8 + 6];
+    (void)cons;
+    if (!(D > 0)) { tp.mode = 0; return fail(PSPH_EINVARIANT, "adjoint: no erupted material among real particles"); }
+    if (n) hipLaunchKernelGGL(k_lam_seed, blocks(n), 256, 0, st, n, c->f.bc, c->pid, c->f.m, c->f.xi, c->f.z, H, D, lam_ptrs(A.L));
+  }
+
+  // ---- reverse sweep, segment by segment ----
+  ThetaGrad th;
+  tp.mode = 2;
+  for (int64_t seg = nseg - 1; seg >= 0 && !rc; seg--) {
+    const int64_t s0 = seg * K, s1 = std::min(nsteps, s0 + K);
+    if (seg != nseg - 1) {
+      rc = snap_load(c, A.ck[seg]);
+      for (int64_t s = s0; s < s1 && !rc; s++) {
+        c->rec = &A.recs[s - s0];
+        rc = psph_step(c, nullptr);
+      }
+      c->rec = nullptr;
+    }
+    for (int64_t s = s1 - 1; s >= s0 && !rc; s--) rc = reverse_step(c, A, A.recs[s - s0], th);
+  }
+  c->rec = nullptr;
+  if (!rc) rc = check_err(c, "adjoint");
+  if (rc) { tp.mode = 0; snap_load(c, A.fin); return rc; }
+
+  // ---- initial state: setup ghosts and the per-particle output ----
+  const Snapshot &S0 = A.ck[0];
+  std::vector<int32_t> bc0(n0);
+  std::vector<double> x0(n0), y0(n0), lam0(6 * std::max<size_t>(n0, 1));
+  std::vector<double> lm(c->next_pid);
+  if (n0) {
+    HIP_TRY(hipMemcpyAsync(bc0.data(), S0.i[0], n0 * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(x0.data(), S0.d[0], n0 * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(y0.data(), S0.d[1], n0 * 8, hipMemcpyDeviceToHost, st));
+    const int fl[6] = {L_Z, L_VZ, L_RHO, L_E, L_T, L_CP};
+    for (int f = 0; f < 6; f++) HIP_TRY(hipMemcpyAsync(lam0.data() + f * n0, A.L[fl[f]], n0 * 8, hipMemcpyDeviceToHost, st));
+  }
+  if (c->next_pid) HIP_TRY(hipMemcpyAsync(lm.data(), A.L[L_M], (size_t)c->next_pid * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  const VentDeriv vd = vent_deriv(k);
+  double lm_erupt = 0;
+  for (size_t q = 0; q < n0; q++) {
+    if (bc0[q] != 0) continue;
+    double one[6];
+    for (int f = 0; f < 6; f++) one[f] = lam0[f * n0 + q];
+    one[0] = 0;                                   // setup ghosts: z independent of the vent velocity
+    // column exit velocity (vel_prof, sph_lib.cc:1223-1243) at the ghost's column centre
+    const double rsq = x0[q] * x0[q] + y0[q] * y0[q];
+    const double vel = k.vent_profile == 0 ? k.vent_v : 2 * k.vent_v * (1 - rsq / (k.vent_r * k.vent_r));
+    ghost_theta(k, vd, vel, 0.0, one, th);
+    lm_erupt += lm[q];
+  }
+  for (size_t q = n0; q < c->next_pid; q++) lm_erupt += lm[q];
+  th.T += lm_erupt * vd.dm_dT;
+  th.ng0 += lm_erupt * vd.dm_dn;
+
+  if (grad_v || grad_sv) {
+    // psph_read order of the state at the start: live particles, THashTable order
+    std::vector<uint32_t> k0(n0), k1(n0), k2(n0);
+    if (n0) {
+      HIP_TRY(hipMemcpy(k0.data(), S0.u[0], n0 * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(k1.data(), S0.u[1], n0 * 4, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(k2.data(), S0.u[2], n0 * 4, hipMemcpyDeviceToHost));
+    }
+    std::vector<uint32_t> live;
+    for (size_t q = 0; q < n0; q++) if (bc0[q] >= 0) live.push_back((uint32_t)q);
+    std::vector<uint32_t> a0(live.size()), a1(live.size()), a2(live.size());
+    for (size_t q = 0; q < live.size(); q++) { a0[q] = k0[live[q]]; a1[q] = k1[live[q]]; a2[q] = k2[live[q]]; }
+    auto perm = canonical_order(k, a0.data(), a1.data(), a2.data(), live.size());
+    if (live.size() > n_cap) { tp.mode = 0; snap_load(c, A.fin); return fail(PSPH_ECAPACITY, "adjoint: %zu particles, gradient buffers hold %zu", live.size(), n_cap); }
+    for (int which = 0; which < 2; which++) {
+      double *dst = which == 0 ? grad_v : grad_sv;
+      if (!dst) continue;
+      std::vector<double> g3(3 * std::max<size_t>(n0, 1));
+      for (int d = 0; d < 3; d++)
+        if (n0) HIP_TRY(hipMemcpy(g3.data() + d * n0, A.L[(which == 0 ? L_VX : L_SVX) + d], n0 * 8, hipMemcpyDeviceToHost));
+      for (size_t r = 0; r < perm.size(); r++) {
+        const uint32_t id = live[perm[r]];
+        for (int d = 0; d < 3; d++) dst[3 * r + d] = g3[d * n0 + id];
+      }
+    }
+    out->n_particles = (int64_t)live.size();
+  }
+  tp.mode = 0;
+  PSPH_TRY(snap_load(c, A.fin));
+  HIP_TRY(hipStreamSynchronize(st));
+  const double t_end = now_ms();
+  out->H = H;
+  out->dH_dvent_v = th.v;
+  out->dH_dvent_T = th.T;
+  out->dH_dng0 = th.ng0;
+  out->dH_dmass_frac = -th.ng0;
+  out->forward_ms = t_fwd - t_start;
+  out->backward_ms = t_end - t_fwd;
+  out->steps = nsteps;
+  out->checkpoints = nseg;
+  return 0;
+}
+
+extern "C" int psph_replay_run(psph_ctx *c, int64_t nsteps) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  Tape &tp = c->tape;
+  if (tp.mode) return fail(PSPH_EINVAL, "replay: a recorded run is in progress");
+  if (nsteps < 0 || nsteps > (int64_t)tp.dt.size()) return fail(PSPH_EINVAL, "replay: %lld steps, %zu recorded", (long long)nsteps, tp.dt.size());
+  if (c->time != tp.t0 || c->step != tp.step0 || c->n != tp.n0)
+    return fail(PSPH_EINVAL, "replay: load the state the recorded run started from (t %g, step %lld, %zu particles)",
+                tp.t0, (long long)tp.step0, tp.n0);
+  tp.mode = 2;
+  tp.k = 0;
+  int rc = 0;
+  for (int64_t s = 0; s < nsteps && !rc; s++) rc = psph_step(c, nullptr);
+  tp.mode = 0;
+  return rc;
+}
+
+extern "C" int psph_set_vent(psph_ctx *c, double vent_v, double vent_T, double ng0) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  psph_config k = c->cfg;
+  k.vent_v = vent_v; k.vent_T = vent_T; k.ng0 = ng0;
+  PSPH_TRY(psph_config_vent(&k));
+  c->cfg = k;
+  return 0;
+}

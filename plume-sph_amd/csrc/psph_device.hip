@@ -22,6 +22,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -228,7 +229,8 @@ __global__ void k_chunks(uint32_t nrows, int nx, const uint32_t *cs, const uint3
 // Pair-pass skeleton
 // ---------------------------------------------------------------------------
 struct PairCommon {
-  const double *x, *y, *z;
+  const double *x, *y, *z, *h;
+  double edge;                // cell edge (0: one cell, all pairs)
   const uint32_t *refb;
   const double *h_list;
   const uint32_t *cellkey;    // sorted cell keys
@@ -246,6 +248,7 @@ struct PairCommon {
 template <int D, class P>
 __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
+  __shared__ float sxf[kWave], syf[kWave], szf[kWave], shf[kWave];
   __shared__ uint32_t srefb[kWave];
   __shared__ typename P::Tile tile;
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
@@ -261,7 +264,8 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   const int lane = threadIdx.x;
   const bool active = (uint32_t)lane < cc;
   const uint32_t i = pc.targets[cf + (active ? lane : 0)];
-  const int cx0 = (int)(pc.cellkey[pc.targets[cf]] % (uint32_t)pc.nx);
+  const uint32_t t0 = pc.targets[cf];
+  const int cx0 = (int)(pc.cellkey[t0] % (uint32_t)pc.nx);
   const int cx1 = (int)(pc.cellkey[pc.targets[cf + cc - 1]] % (uint32_t)pc.nx);
   const int cy = (int)(row % (uint32_t)pc.ny), cz = (int)(row / (uint32_t)pc.ny);
 
@@ -271,9 +275,23 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   typename P::Tgt t;
   P::load_target(a, i, t);
 
+  // Phase 1 runs in FP32 on coordinates relative to the chunk's first target
+  // (3D cell grid).  Every staged candidate lies within dlim of that origin
+  // (else its tile takes the exact path), so |error of max_k |dx_k|| and the
+  // rounding of the FP32 thresholds stay below m = dlim 2^-19; a comparison
+  // decided by more than m is exact, and the rest re-run the FP64 test.
   const int xlo = max(cx0 - pc.r, 0), xhi = min(cx1 + pc.r, pc.nx - 1);
+  const double dlim = (double)(max(xhi - xlo + 1, 2 * pc.r + 1) + 1) * pc.edge;
+  const double ox = pc.x[t0], oy = pc.y[t0], oz = pc.z[t0];
+  const bool tfar = active && !(fmax(fabs(xi[0] - ox), fmax(fabs(xi[1] - oy), fabs(xi[2] - oz))) < dlim);
+  const bool f32 = D == 3 && pc.edge > 0 && !__any(tfar);
+  const float m = (float)(dlim * 0x1p-19);
+  const float cutf = (float)pc.cut.cut;
+  const float xif[3] = {(float)(xi[0] - ox), (float)(xi[1] - oy), (float)(xi[2] - oz)};
+  const float lff = (float)lf_i;
   for (int pass = 0; pass < P::npasses(a); pass++) {
     P::begin_pass(a, t, pass);
+    const float sup_lo = (float)t.supp - m, sup_hi = (float)t.supp + m;
     for (int dz = -pc.r; dz <= pc.r; dz++) {
       const int zz = cz + dz;
       if (zz < 0 || zz >= pc.nz) continue;
@@ -284,31 +302,54 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
         const uint32_t lo = pc.cs[rbase + xlo], hi = pc.cs[rbase + xhi + 1];
         for (uint32_t base = lo; base < hi; base += kWave) {
           const uint32_t j = base + lane;
+          bool far = false;
           if (j < hi) {
-            sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+            const double xj = pc.x[j], yj = pc.y[j], zj = pc.z[j];
+            sx[lane] = xj; sy[lane] = yj; sz[lane] = zj;
             srefb[lane] = pc.refb[j];
+            if (f32) {
+              const double rx = xj - ox, ry = yj - oy, rz = zj - oz;
+              far = !(fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) < dlim);
+              sxf[lane] = (float)rx; syf[lane] = (float)ry; szf[lane] = (float)rz;
+              shf[lane] = (float)pc.h[j];
+            }
             P::stage(a, j, tile, lane);
           }
+          const bool exact_tile = !f32 || __any(far);
           __syncthreads();
           const uint32_t cnt = min((uint32_t)kWave, hi - base);
           if (active && P::want(t)) {
-            // phase 1: exact FP64 predicate -> per-lane hit mask
-            uint64_t mask = 0;
-            for (uint32_t q = 0; q < cnt; q++) {
-              double dx[3];
-              dx[0] = xi[0] - sx[q];
-              double amax = fabs(dx[0]);
-              if (D == 3) {
-                dx[1] = xi[1] - sy[q];
-                dx[2] = xi[2] - sz[q];
-                amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
-              }
+            // exact FP64 predicate for candidate q
+            auto exact = [&](uint32_t q) -> bool {
+              double amax = fabs(xi[0] - sx[q]);
+              if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
               // outside the kernel box (exact, see CutTest)
-              if (pc.cut.outside(amax, P::hker(a, t, tile, q))) continue;
+              if (pc.cut.outside(amax, P::hker(a, t, tile, q))) return false;
               // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
-              const bool in = (pc.list_all || srefb[q] == rb_i || amax <= lf_i) &&
-                              amax <= t.supp && P::extra(a, t, tile, q, base + q, i);
-              if (in) mask |= 1ull << q;
+              return (pc.list_all || srefb[q] == rb_i || amax <= lf_i) && amax <= t.supp &&
+                     P::extra(a, t, tile, q, base + q, i);
+            };
+            uint64_t mask = 0;
+            if (exact_tile) {
+              for (uint32_t q = 0; q < cnt; q++)
+                if (exact(q)) mask |= 1ull << q;
+            } else {
+              uint64_t amb = 0;
+              for (uint32_t q = 0; q < cnt; q++) {
+                const float am = fmaxf(fabsf(xif[0] - sxf[q]), fmaxf(fabsf(xif[1] - syf[q]), fabsf(xif[2] - szf[q])));
+                const float tb = cutf * P::hker_f(t, shf[q]);
+                const bool same = pc.list_all || srefb[q] == rb_i;
+                const bool ex = P::extra(a, t, tile, q, base + q, i);
+                const bool in = ex && am <= tb - m && (same || am <= lff - m) && am <= sup_lo;
+                const bool out = !ex || am > tb + m || (!same && am > lff + m) || am > sup_hi;
+                if (in) mask |= 1ull << q;
+                else if (!out) amb |= 1ull << q;
+              }
+              while (amb) {
+                const uint32_t q = (uint32_t)__builtin_ctzll(amb);
+                amb &= amb - 1;
+                if (exact(q)) mask |= 1ull << q;
+              }
             }
             // phase 2: each lane walks its own hits
             while (mask) {
@@ -384,6 +425,7 @@ struct RhsPass {
   };
   struct Tgt {
     double hi, supp, rv[3], re;
+    float hif;
     R rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3];
     unsigned npairs;
     bool bad;
@@ -394,6 +436,7 @@ struct RhsPass {
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i];
+    t.hif = (float)t.hi;
     t.supp = a.cutoff2 * t.hi;
     const double rho = a.rho[i];
     t.bad = !(rho > 0);
@@ -414,6 +457,7 @@ struct RhsPass {
   }
   // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2); j != i (dW(0) = 0)
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
+  __device__ static float hker_f(const Tgt &t, float hj) { return 0.5f * (t.hif + hj); }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -514,6 +558,7 @@ struct DensArgs {
   int pghost, energy_smooth;
   EosConst eos;
   CutTest cut;
+  double *o_wn;            // optional: energy-smoothing normalisation (adjoint recompute)
 };
 
 template <class R>
@@ -524,13 +569,14 @@ struct DensPass {
     R m[kWave], rho[kWave], e[kWave];
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
-  struct Tgt { double hi, supp, r1, r2, wn, en; };
+  struct Tgt { double hi, supp, r1, r2, wn, en; float hif; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi;
+    t.hif = (float)t.hi;
     t.r1 = t.r2 = t.wn = t.en = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
@@ -541,6 +587,7 @@ struct DensPass {
   // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
   // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
+  __device__ static float hker_f(const Tgt &t, float hj) { return 0.5f * (t.hif + hj); }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -583,6 +630,7 @@ struct DensPass {
     if (a.energy_smooth) {
       if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
       e = t.en / t.wn;
+      if (a.o_wn) a.o_wn[i] = t.wn;
     }
     if (!(density > 0) || !(xi <= 1)) { raise_err(err, PSPH_EINVARIANT, i); return; }
     a.o_rho[i] = density;
@@ -603,6 +651,7 @@ struct VsmArgs {
   const int32_t *bc;
   double *svx, *svy, *svz;
   double cutoff, cutoff2;
+  double *o_wn;            // optional: the filter's normalisation sum (adjoint recompute)
 };
 
 template <class R>
@@ -635,6 +684,7 @@ struct VsmPass {
     s.ok[l] = contr_vel(a.bc[j]);
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
+  __device__ static float hker_f(const Tgt &t, float) { return (float)t.hi; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -656,6 +706,7 @@ struct VsmPass {
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
     const double iw = 1.0 / t.wn;
+    if (a.o_wn) a.o_wn[i] = t.wn;
     a.svx[i] = t.v64[0] + t.dv[0] * iw;
     if (a.svy) { a.svy[i] = t.v64[1] + t.dv[1] * iw; a.svz[i] = t.v64[2] + t.dv[2] * iw; }
   }
@@ -682,10 +733,11 @@ struct AdaptPass {
     R m[kWave];
     int ok[kWave];
   };
-  struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; bool done; };
+  struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; float Hsf; bool done; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &a, Tgt &t, int) {
     t.Hs = t.hcur * a.c_smooth;
+    t.Hsf = (float)t.Hs;
     const double iHs = 1.0 / t.Hs;
     t.iHs = (R)iHs;
     t.supp = a.cutoff2 * t.Hs;
@@ -702,6 +754,7 @@ struct AdaptPass {
     s.ok[l] = contr_dens(a.pghost, a.bc[j]);
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.Hs; }
+  __device__ static float hker_f(const Tgt &t, float) { return t.Hsf; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -754,6 +807,7 @@ struct ListPass {
   __device__ static void stage(const Args &, uint32_t, Tile &, int) {}
   __device__ static double bound(const Args &, const Tgt &, const Tile &, uint32_t) { return HUGE_VAL; }
   __device__ static double hker(const Args &, const Tgt &, const Tile &, uint32_t) { return HUGE_VAL; }
+  __device__ static float hker_f(const Tgt &, float) { return HUGE_VALF; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t, uint32_t) { return true; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &, uint32_t, uint32_t j, uint32_t,
@@ -778,12 +832,22 @@ struct Bound : P {
 // ---------------------------------------------------------------------------
 // Position update + escape (update_pos.cc:34-168, bucket.cc:435-452)
 // ---------------------------------------------------------------------------
+// Schedule of a recorded run: tmode 1 records, by particle id, the tape step
+// at which an erupt ghost turns real and a real particle escapes; tmode 2
+// replays those events instead of testing positions.
+struct PosTape {
+  int mode;
+  int32_t k;
+  const uint32_t *pid;
+  int32_t *tr_step, *esc_step;
+};
+
 __global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double dt, double *x,
                              double *y, double *z, const double *vx, const double *vy,
                              const double *vz, const double *svx, const double *svy,
                              const double *svz, int32_t *bc, double *h, int32_t *inv,
                              double ground, double h_new, double lo0, double lo1, double lo2,
-                             double hi0, double hi1, double hi2, unsigned long long *dead) {
+                             double hi0, double hi1, double hi2, unsigned long long *dead, PosTape tp) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int b = bc[i];
@@ -799,15 +863,35 @@ __global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double
     if (dim == 3) {
       y[i] = y[i] + dt * vy[i];
       z[i] = z[i] + dt * vz[i];
-      if (z[i] >= ground) { b = 100; bc[i] = 100; h[i] = h_new; inv[i] = 2; }
+      const bool turn = tp.mode == 2 ? tp.tr_step[tp.pid[i]] == tp.k : z[i] >= ground;
+      if (turn) {
+        b = 100; bc[i] = 100; h[i] = h_new; inv[i] = 2;
+        if (tp.mode == 1) tp.tr_step[tp.pid[i]] = tp.k;
+      }
     }
   }
   if (dim == 3 && (b == 100 || b == 99)) {
     const double TINY = 1.0E-08;
     bool esc = (x[i] - lo0) < -TINY || (y[i] - lo1) < -TINY || (z[i] - lo2) < -TINY ||
                (x[i] - hi0) > TINY || (y[i] - hi1) > TINY || (z[i] - hi2) > TINY;
-    if (esc) { bc[i] = -1; atomicAdd(dead, 1ull); }
+    if (tp.mode == 2) esc = tp.esc_step[tp.pid[i]] == tp.k;
+    if (esc) {
+      bc[i] = -1; atomicAdd(dead, 1ull);
+      if (tp.mode == 1) tp.esc_step[tp.pid[i]] = tp.k;
+    }
   }
+}
+
+__global__ void k_iota_u32(uint32_t *p, uint32_t first, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = first + i;
+}
+// tab[id] = h (record) or h = tab[id] (replay) for the particles present
+__global__ void k_h_by_id(uint32_t n, const uint32_t *pid, double *h, double *tab, int replay) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (replay) h[i] = tab[pid[i]];
+  else tab[pid[i]] = h[i];
 }
 
 // fixed-order block partial sums (deterministic)
@@ -861,6 +945,47 @@ struct DevFields {
 #undef I_
 };
 
+// Frozen schedule of a recorded run (the adjoint and its finite-difference
+// checks differentiate the step map with these discrete choices held fixed).
+struct Tape {
+  int mode = 0;                             // 0 off, 1 record, 2 replay
+  int64_t k = 0;                            // steps taken since the start of the tape
+  double t0 = 0;
+  int64_t step0 = 0;
+  size_t n0 = 0;
+  std::vector<double> dt;                   // per step, after TimeProps::incrtime
+  std::vector<int> hslot;                   // per step: adapted-h table, or -1
+  std::vector<double *> htab;               // smoothing lengths after adaptation, by particle id
+  std::vector<std::vector<int32_t>> inlet;  // per step: (q, n) per vent column
+  int32_t *tr_step = nullptr, *esc_step = nullptr;   // by id, -1 = never
+  size_t id_cap = 0;
+};
+
+// What the reverse sweep needs of one step, in that step's cell order.
+enum RecField {
+  RF_X, RF_Y, RF_Z, RF_H, RF_M, RF_RHO, RF_P, RF_C, RF_T, RF_CP, RF_SVX, RF_SVY, RF_SVZ,
+  RF_VX, RF_VY, RF_VZ, RF_E, RF_HLIST,                  // state after sort and h update
+  RF_V1X, RF_V1Y, RF_V1Z, RF_E1,                         // after the RHS update
+  RF_RHO2, RF_E2, RF_XI2, RF_WNE,                        // after the density pass
+  RF_SV2X, RF_SV2Y, RF_SV2Z, RF_WNV,                     // after the velocity filter
+  RF_N
+};
+struct StepRec {
+  size_t n = 0, cap = 0;
+  double *d[RF_N] = {};
+  uint32_t *pid = nullptr, *refb = nullptr, *ckey = nullptr;
+  int32_t *bc = nullptr, *phase = nullptr;
+  uint32_t *cs = nullptr;
+  size_t cs_cap = 0;
+  GridDesc grid{};
+  int r_main = 1;
+  double dt = 0, time = 0;
+  int erupting = 0;
+  uint32_t inj_lo = 0, inj_hi = 0;          // ids of the particles this step injected
+  std::vector<double> inj_vel;              // their column exit velocity
+};
+struct Adjoint;
+
 struct psph_ctx {
   psph_config cfg{};
   hipStream_t stream = nullptr;
@@ -892,6 +1017,12 @@ struct psph_ctx {
   int64_t last_injected = 0, last_removed = 0;
   unsigned long long last_pairs = 0;
   HostParticles inj;
+  // persistent particle ids: load order, then injection order
+  uint32_t *pid = nullptr, *pid_sp = nullptr;
+  uint32_t next_pid = 0;
+  Tape tape;
+  StepRec *rec = nullptr;      // set: psph_step captures the step for the reverse sweep
+  Adjoint *adj = nullptr;
   // per-stage HIP-event timing
   bool prof = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -978,6 +1109,8 @@ extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **o
   if (!rc) rc = dmalloc((void **)&c->usp, capacity * 4);
   if (!rc) rc = dmalloc((void **)&c->isp, capacity * 4);
   if (!rc) rc = dmalloc((void **)&c->h_list, capacity * 8);
+  if (!rc) rc = dmalloc((void **)&c->pid, capacity * 4);
+  if (!rc) rc = dmalloc((void **)&c->pid_sp, capacity * 4);
   uint32_t **u32s[] = {&c->refb, &c->key_a, &c->key_b, &c->idx_a, &c->idx_b, &c->targets};
   for (auto pp : u32s)
     if (!rc) rc = dmalloc((void **)pp, capacity * 4);
@@ -995,14 +1128,21 @@ extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **o
   return 0;
 }
 
+static void adjoint_free(psph_ctx *c);
+static void tape_clear(Tape &tp);
+
 extern "C" int psph_destroy(psph_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
+  adjoint_free(c);
+  tape_clear(c->tape);
+  if (c->tape.tr_step) hipFree(c->tape.tr_step);
+  if (c->tape.esc_step) hipFree(c->tape.esc_step);
 #define F_(a) if (c->f.a) hipFree(c->f.a);
   PSPH_DFIELDS(F_) PSPH_UFIELDS(F_) PSPH_IFIELDS(F_)
 #undef F_
   for (auto p : c->dsp) if (p) hipFree(p);
-  void *ptrs[] = {c->usp, c->isp, c->h_list, c->refb, c->key_a, c->key_b, c->idx_a, c->idx_b,
+  void *ptrs[] = {c->usp, c->isp, c->h_list, c->pid, c->pid_sp, c->refb, c->key_a, c->key_b, c->idx_a, c->idx_b,
                   c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->crow, c->cs,
                   c->row_chunks, c->row_off, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part};
   for (auto p : ptrs) if (p) hipFree(p);
@@ -1033,6 +1173,9 @@ extern "C" int psph_load(psph_ctx *c, const psph_soa *host, size_t n, double t0,
   P.from_soa(host, n);
   PSPH_TRY(upload(c, P, 0));
   c->n = n;
+  if (n) hipLaunchKernelGGL(k_iota_u32, blocks(n), 256, 0, c->stream, c->pid, 0u, (uint32_t)n);
+  c->next_pid = (uint32_t)n;
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 3, 0ull, 1);
   c->time = t0;
   c->step = step0;
   c->ioutput = 0;
@@ -1210,6 +1353,8 @@ extern "C" int psph_neighbors(psph_ctx *c) {
 #define G_(a) { hipLaunchKernelGGL(k_gather<int32_t>, blocks(n), 256, 0, s, n, c->idx_b, c->f.a, c->isp); std::swap(c->f.a, c->isp); }
     PSPH_IFIELDS(G_)
 #undef G_
+    hipLaunchKernelGGL(k_gather<uint32_t>, blocks(n), 256, 0, s, n, c->idx_b, c->pid, c->pid_sp);
+    std::swap(c->pid, c->pid_sp);
     std::swap(c->key_a, c->key_b);   // key_a: sorted keys
   }
   // drop removed particles (their key == ncells sorts them to the tail)
@@ -1261,7 +1406,8 @@ static CutTest cut_test(double cut) {
 
 static PairCommon make_common(psph_ctx *c, int r) {
   PairCommon pc;
-  pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z;
+  pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z; pc.h = c->f.h;
+  pc.edge = c->grid.inv_e > 0 ? 1.0 / c->grid.inv_e : 0.0;
   pc.refb = c->refb; pc.h_list = c->h_list;
   pc.cellkey = c->key_a; pc.cs = c->cs; pc.targets = c->targets;
   pc.cfirst = c->cfirst; pc.ccount = c->ccount; pc.crow = c->crow; pc.nchunks = c->nchunks;
@@ -1310,6 +1456,16 @@ static int reduce_hmax(psph_ctx *c, double *out) {
   return 0;
 }
 
+// the later passes of a step use the new h: widen the cell reach
+static void set_reach(psph_ctx *c, double hmax) {
+  const psph_config &k = c->cfg;
+  c->h_max = hmax;
+  if (!(k.list_all || k.dim == 1)) {
+    const double E = 1.0 / c->grid.inv_e;
+    c->r_main = std::max(1, (int)std::ceil(k.cutoff * hmax * (1 + 1e-9) / E - 1e-12));
+  }
+}
+
 extern "C" int psph_adapt_h(psph_ctx *c) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   if (!c->grid_valid) return fail(PSPH_EINVAL, "adapt_h: call psph_neighbors first");
@@ -1338,9 +1494,18 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
     PSPH_TRY(reduce_hmax(c, &hmax));
   }
   PSPH_TRY(check_err(c, "adapt_h"));
-  // the later passes of this step use the new h: widen the cell reach
-  c->h_max = hmax;
-  if (!all) c->r_main = std::max(1, (int)std::ceil(k.cutoff * hmax * (1 + 1e-9) / E - 1e-12));
+  set_reach(c, hmax);
+  return 0;
+}
+
+// replays (or records) the smoothing lengths of an adaptation step by id
+static int h_by_id(psph_ctx *c, double *tab, int replay) {
+  if (c->n) hipLaunchKernelGGL(k_h_by_id, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->pid, c->f.h, tab, replay);
+  HIP_TRY(hipGetLastError());
+  if (!replay) return 0;
+  double hmax;
+  PSPH_TRY(reduce_hmax(c, &hmax));
+  set_reach(c, hmax);
   return 0;
 }
 
@@ -1392,6 +1557,7 @@ extern "C" int psph_density(psph_ctx *c) {
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.e_ratio = k.e_smooth_ratio; a.msfrc_thresh = k.msfrc_thresh;
   a.pghost = k.pghost_contrib; a.energy_smooth = k.energy_smooth; a.eos = eos_const(k);
   a.cut = cut_test(k.cutoff);
+  a.o_wn = c->rec ? c->rec->d[RF_WNE] : nullptr;
   {
     StageTimer tm(c, 4);
     if (n) launch_form<DensPass>(c, make_common(c, c->r_main), a);
@@ -1411,6 +1577,7 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   a.vy = k.dim == 3 ? c->f.vy : nullptr; a.vz = k.dim == 3 ? c->f.vz : nullptr;
   a.bc = c->f.bc; a.svx = c->f.svx; a.svy = k.dim == 3 ? c->f.svy : nullptr; a.svz = k.dim == 3 ? c->f.svz : nullptr;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2;
+  a.o_wn = c->rec ? c->rec->d[RF_WNV] : nullptr;
   {
     StageTimer tm(c, 5);
     if (c->n) launch_form<VsmPass>(c, make_common(c, c->r_main), a);
@@ -1429,13 +1596,15 @@ extern "C" int psph_update_pos(psph_ctx *c, double dt) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   const psph_config &k = c->cfg;
   uint32_t n = (uint32_t)c->n;
+  PosTape tp{c->tape.mode, (int32_t)c->tape.k, c->pid, c->tape.tr_step, c->tape.esc_step};
+  if (tp.mode && (!tp.tr_step || c->next_pid > c->tape.id_cap)) return fail(PSPH_EINVAL, "update_pos: tape has no id table");
   StageTimer tm(c, 6);
   if (n)
     hipLaunchKernelGGL(k_update_pos, blocks(n), 256, 0, c->stream, n, k.dim, k.lans, iferupt(c), dt,
                        c->f.x, c->f.y, c->f.z, c->f.vx, c->f.vy, c->f.vz, c->f.svx, c->f.svy, c->f.svz,
                        c->f.bc, c->f.h, c->f.inv, k.dom_lo[2], k.ratio_sml_dx * k.h_air,
                        k.dom_lo[0], k.dom_lo[1], k.dom_lo[2], k.dom_hi[0], k.dom_hi[1], k.dom_hi[2],
-                       c->d_u64 + 3);
+                       c->d_u64 + 3, tp);
   tm.stop();
   HIP_TRY(hipGetLastError());
   unsigned long long dead = 0;
@@ -1446,20 +1615,71 @@ extern "C" int psph_update_pos(psph_ctx *c, double dt) {
   return 0;
 }
 
+// id-indexed event tables of the tape, grown to hold `need` ids (new: -1)
+static int tape_ids(psph_ctx *c, size_t need) {
+  Tape &tp = c->tape;
+  if (need <= tp.id_cap && tp.tr_step) return 0;
+  size_t cap = std::max(need, std::max(2 * tp.id_cap, c->cap));
+  int32_t *a = nullptr, *b = nullptr;
+  PSPH_TRY(dmalloc((void **)&a, cap * 4));
+  if (dmalloc((void **)&b, cap * 4)) { hipFree(a); return PSPH_ENOMEM; }
+  hipLaunchKernelGGL(k_fill_i32, blocks(cap), 256, 0, c->stream, a, -1, (int)cap);
+  hipLaunchKernelGGL(k_fill_i32, blocks(cap), 256, 0, c->stream, b, -1, (int)cap);
+  if (tp.tr_step) {
+    HIP_TRY(hipMemcpyAsync(a, tp.tr_step, tp.id_cap * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(b, tp.esc_step, tp.id_cap * 4, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipFree(tp.tr_step);
+    hipFree(tp.esc_step);
+  }
+  tp.tr_step = a;
+  tp.esc_step = b;
+  tp.id_cap = cap;
+  return 0;
+}
+
+static void tape_clear(Tape &tp) {
+  for (double *p : tp.htab) if (p) hipFree(p);
+  tp.htab.clear();
+  tp.dt.clear();
+  tp.hslot.clear();
+  tp.inlet.clear();
+  tp.k = 0;
+  tp.mode = 0;
+}
+
+// captures the step's intermediates into c->rec (defined with the adjoint)
+static int rec_capture(psph_ctx *c, int stage, double dt);
+
 extern "C" int psph_inject(psph_ctx *c, double dt, int64_t *added) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   if (added) *added = 0;
   c->last_injected = 0;
-  if (c->cfg.dim != 3 || !iferupt(c)) return 0;
+  Tape &tp = c->tape;
+  std::vector<int32_t> *rec = nullptr;
+  const std::vector<int32_t> *rep = nullptr;
+  if (tp.mode == 1) { tp.inlet.emplace_back(); rec = &tp.inlet.back(); }
+  if (tp.mode == 2) {
+    if (tp.k >= (int64_t)tp.inlet.size()) return fail(PSPH_EINVAL, "inject: replay past the recorded run");
+    rep = &tp.inlet[tp.k];
+  }
   c->inj.resize(0);
-  PSPH_TRY(inject_host(c->cfg, c->time, dt, c->step, c->inj));
+  if (c->cfg.dim != 3 || !iferupt(c)) {
+    if (rep && !rep->empty()) return fail(PSPH_EINVAL, "inject: recorded run erupted at this step");
+    return 0;
+  }
+  PSPH_TRY(inject_host(c->cfg, c->time, dt, c->step, c->inj, rec, rep ? rep->data() : nullptr,
+                       rep ? rep->size() : 0));
   size_t m = c->inj.size();
   if (c->n + m > c->cap)
     return fail(PSPH_ECAPACITY, "inject: %zu + %zu particles exceed capacity %zu", c->n, m, c->cap);
+  if (tp.mode && c->next_pid + m > tp.id_cap) PSPH_TRY(tape_ids(c, c->next_pid + m));
   {
     StageTimer tm(c, 7);
     PSPH_TRY(upload(c, c->inj, c->n));
   }
+  if (m) hipLaunchKernelGGL(k_iota_u32, blocks(m), 256, 0, c->stream, c->pid + c->n, c->next_pid, (uint32_t)m);
+  c->next_pid += (uint32_t)m;
   c->n += m;
   c->last_injected = (int64_t)m;
   if (added) *added = (int64_t)m;
@@ -1470,17 +1690,51 @@ extern "C" int psph_inject(psph_ctx *c, double dt, int64_t *added) {
 extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   const psph_config &k = c->cfg;
+  Tape &tp = c->tape;
   double dt;
-  PSPH_TRY(psph_timestep(c, &dt));
-  PSPH_TRY(psph_advance_time(c, &dt));
+  if (tp.mode == 2) {
+    if (tp.k >= (int64_t)tp.dt.size()) return fail(PSPH_EINVAL, "replay: past the end of the recorded run");
+    dt = tp.dt[tp.k];
+    c->time += dt;
+    c->step++;
+  } else {
+    PSPH_TRY(psph_timestep(c, &dt));
+    PSPH_TRY(psph_advance_time(c, &dt));
+    if (tp.mode == 1) tp.dt.push_back(dt);
+  }
   PSPH_TRY(psph_neighbors(c));
-  if (k.sml_update_int > 0 && c->step % k.sml_update_int == 0) PSPH_TRY(psph_adapt_h(c));
+  if (tp.mode == 2) {
+    const int slot = tp.hslot[tp.k];
+    if (slot >= 0) PSPH_TRY(h_by_id(c, tp.htab[slot], 1));
+  } else if (k.sml_update_int > 0 && c->step % k.sml_update_int == 0) {
+    PSPH_TRY(psph_adapt_h(c));
+    if (tp.mode == 1) {
+      double *tab = nullptr;
+      PSPH_TRY(dmalloc((void **)&tab, (size_t)c->next_pid * 8));
+      tp.htab.push_back(tab);
+      tp.hslot.push_back((int)tp.htab.size() - 1);
+      PSPH_TRY(h_by_id(c, tab, 0));
+    }
+  } else if (tp.mode == 1) {
+    tp.hslot.push_back(-1);
+  }
+  if (c->rec) PSPH_TRY(rec_capture(c, 0, dt));
   PSPH_TRY(psph_rhs(c, dt));
+  if (c->rec) PSPH_TRY(rec_capture(c, 1, dt));
   PSPH_TRY(psph_density(c));
+  if (c->rec) PSPH_TRY(rec_capture(c, 2, dt));
   if (k.lans) PSPH_TRY(psph_vsmooth(c));
+  if (c->rec) PSPH_TRY(rec_capture(c, 3, dt));
   PSPH_TRY(psph_update_pos(c, dt));
   int64_t added = 0;
+  const uint32_t id0 = c->next_pid;
   PSPH_TRY(psph_inject(c, dt, &added));
+  if (c->rec) {
+    c->rec->inj_lo = id0;
+    c->rec->inj_hi = c->next_pid;
+    c->rec->inj_vel.assign(c->inj.vz.begin(), c->inj.vz.end());
+  }
+  if (tp.mode) tp.k++;
   // TimeProps::ifoutput (properties.h:104-120)
   if (c->time >= c->next_output || c->time == k.t_end) {
     c->ioutput++;
@@ -1616,3 +1870,5 @@ extern "C" int psph_profile_read(psph_ctx *c, double ms[PSPH_NSTAGES], int64_t l
   }
   return 0;
 }
+
+#include "psph_adjoint.inl"

@@ -255,9 +255,12 @@ static void new_erupt(const psph_config &cfg, HostParticles &P, double x, double
   P.gam[i] = cfg.vent_gamma;
 }
 
-int inject_host(const psph_config &cfg, double time, double dt, int64_t step, HostParticles &out) {
+int inject_host(const psph_config &cfg, double time, double dt, int64_t step, HostParticles &out,
+                std::vector<int32_t> *rec, const int32_t *rep, size_t rep_len) {
   double t_total = time - dt;   // time of the previous step (setup_erupt.cc:451)
   double rvsq = cfg.vent_r * cfg.vent_r;
+  size_t col = 0;
+  int rc = 0;
   vent_columns(cfg, [&](double x, double y) {
     double dist = 0;
     dist += x * x;
@@ -265,12 +268,29 @@ int inject_host(const psph_config &cfg, double time, double dt, int64_t step, Ho
     if (!(dist <= rvsq)) return;
     double vel = vel_prof(cfg, cfg.vent_r, dist, cfg.vent_v);
     double t_each = cfg.vent_sml / vel;
-    double t_add = std::fmod(t_total, t_each) + dt;
-    int n = (int)std::floor(t_add / t_each);
-    t_add = t_add - n * t_each;
+    double t_add;
+    int n;
+    if (rep) {
+      if (2 * col + 1 >= rep_len) { rc = PSPH_EINVAL; return; }
+      double q = rep[2 * col];
+      n = rep[2 * col + 1];
+      t_add = (t_total - q * t_each) + dt;
+      t_add = t_add - n * t_each;
+    } else {
+      double r = std::fmod(t_total, t_each);
+      t_add = r + dt;
+      n = (int)std::floor(t_add / t_each);
+      t_add = t_add - n * t_each;
+      if (rec) {
+        rec->push_back((int32_t)std::llround((t_total - r) / t_each));
+        rec->push_back(n);
+      }
+    }
+    col++;
     for (int i = 0; i < n; i++)
       new_erupt(cfg, out, x, y, cfg.vent_bot + (t_add + i * t_each) * vel, (uint32_t)step, vel);
   });
+  if (rc) return fail(rc, "inject: recorded inlet schedule has fewer columns than the vent");
   return 0;
 }
 
@@ -677,6 +697,7 @@ extern "C" size_t psph_abi_sizeof(int which) {
     case 0: return sizeof(psph_config);
     case 1: return sizeof(psph_soa);
     case 2: return sizeof(psph_step_out);
+    case 3: return sizeof(psph_adjoint_out);
     default: return 0;
   }
 }

@@ -40,7 +40,12 @@ void eos(const psph_config &cfg, double rho, double e, double xi, double &p, dou
          double &T, double &gam, double &cp);
 
 // Inlet for one step (setup_erupt.cc:389-670): appends new erupt ghosts.
+// With rec, the per-column whole-period count q of fmod(t, t_each) and the
+// spawn count n are appended (q, n) in scan order; with rep, they are taken
+// from there instead (the schedule of a recorded run, for the adjoint and its
+// finite-difference checks), and the spawn time is t - q t_each + dt - n t_each.
 int inject_host(const psph_config &cfg, double time, double dt, int64_t step,
-                HostParticles &out);
+                HostParticles &out, std::vector<int32_t> *rec = nullptr,
+                const int32_t *rep = nullptr, size_t rep_len = 0);
 
 }  // namespace psph

@@ -84,6 +84,13 @@ class StepOut(C.Structure):
                 ("n_removed", C.c_int64), ("n_pairs", C.c_int64)]
 
 
+class AdjointOut(C.Structure):
+    _fields_ = [("H", C.c_double), ("dH_dvent_v", C.c_double), ("dH_dvent_T", C.c_double),
+                ("dH_dng0", C.c_double), ("dH_dmass_frac", C.c_double), ("forward_ms", C.c_double),
+                ("backward_ms", C.c_double), ("steps", C.c_int64), ("checkpoints", C.c_int64),
+                ("n_particles", C.c_int64), ("reserved", C.c_double * 4)]
+
+
 class Meteo(C.Structure):
     _fields_ = [("rows", C.c_int32), ("nprop", C.c_int32),
                 ("z", C.POINTER(C.c_double)), ("data", C.POINTER(C.c_double))]
@@ -181,11 +188,14 @@ def lib() -> C.CDLL:
         "psph_abi_sizeof": (C.c_size_t, [C.c_int]),
         "psph_profile": (C.c_int, [P, C.c_int]),
         "psph_profile_read": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "psph_adjoint_run": (C.c_int, [P, C.c_int64, C.c_int64, C.POINTER(AdjointOut), C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_size_t]),
+        "psph_replay_run": (C.c_int, [P, C.c_int64]),
+        "psph_set_vent": (C.c_int, [P, C.c_double, C.c_double, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype, f.argtypes = res, args
-    for which, st in enumerate((Config, SoA, StepOut)):
+    for which, st in enumerate((Config, SoA, StepOut, AdjointOut)):
         if L.psph_abi_sizeof(which) != C.sizeof(st):
             raise PsphError(-1, f"ABI mismatch for {st.__name__}: C {L.psph_abi_sizeof(which)} vs ctypes {C.sizeof(st)}")
     _lib = L
@@ -417,3 +427,32 @@ class Solver:
         H = C.c_double()
         check(lib().psph_diag_height(self._ctx, C.byref(H)))
         return H.value
+
+    # -- adjoint -----------------------------------------------------------
+    def adjoint_run(self, nsteps: int, ckpt_every: int, grads: bool = True) -> dict:
+        """Runs nsteps steps and the reverse sweep (psph_adjoint_run).
+
+        Returns H, its derivatives with respect to the vent velocity, vent
+        temperature, gas mass fraction ng0 and solid mass fraction 1 - ng0,
+        the timings, and (grads=True) dH/dv and dH/dsv of the starting state
+        as (n, 3) arrays in read order."""
+        o = AdjointOut()
+        n0, _ = self.count() if grads else (0, 0)
+        gv = np.zeros((max(n0, 1), 3))
+        gs = np.zeros((max(n0, 1), 3))
+        ptr = (lambda a: a.ctypes.data_as(C.POINTER(C.c_double))) if grads else (lambda a: None)
+        check(lib().psph_adjoint_run(self._ctx, nsteps, ckpt_every, C.byref(o), ptr(gv), ptr(gs), n0))
+        r = {k: getattr(o, k) for k, _ in AdjointOut._fields_ if k != "reserved"}
+        if grads:
+            r["grad_v"] = gv[:o.n_particles]
+            r["grad_sv"] = gs[:o.n_particles]
+        return r
+
+    def replay(self, nsteps: int):
+        """nsteps steps with the frozen schedule of the last adjoint_run."""
+        check(lib().psph_replay_run(self._ctx, nsteps))
+
+    def set_vent(self, vent_v: float, vent_T: float, ng0: float):
+        check(lib().psph_set_vent(self._ctx, vent_v, vent_T, ng0))
+        self.cfg.vent_v, self.cfg.vent_T, self.cfg.ng0 = vent_v, vent_T, ng0
+        check(lib().psph_config_vent(C.byref(self.cfg)))
