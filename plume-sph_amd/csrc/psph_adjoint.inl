@@ -169,9 +169,10 @@ __global__ void k_adj_rhs_pre(uint32_t n, const int32_t *bc, double dt, double g
 // ---------------------------------------------------------------------------
 struct AdjCommon {
   const double *x, *y, *z, *h, *h_list;
-  const uint32_t *refb, *cellkey, *cs, *rows, *cfirst, *ccount, *crow, *nchunks;
+  const uint32_t *refb, *rows, *cfirst, *ccount, *nchunks;
   const int32_t *bc;
-  int nx, ny, nz, r, list_all;
+  Traversal tr;
+  int list_all;
   double list_factor, cutoff2;
   CutTest cut;
 };
@@ -188,13 +189,10 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
   if (local >= per) return;
   const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
   if (chunk >= nch) return;
-  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk], row = pc.crow[chunk];
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
   const int lane = threadIdx.x;
   const bool active = (uint32_t)lane < cc;
   const uint32_t i = pc.rows[cf + (active ? lane : 0)];
-  const int cx0 = (int)(pc.cellkey[pc.rows[cf]] % (uint32_t)pc.nx);
-  const int cx1 = (int)(pc.cellkey[pc.rows[cf + cc - 1]] % (uint32_t)pc.nx);
-  const int cy = (int)(row % (uint32_t)pc.ny), cz = (int)(row / (uint32_t)pc.ny);
 
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
   const uint32_t rb_i = pc.refb[i];
@@ -204,17 +202,17 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
   const bool tgt_i = need_neigh(pc.bc[i]);
   typename P::Row t;
   P::load_row(a, i, t);
+  const ChunkBox box = chunk_box(pc.tr, xi);
+  const GridDesc &g = pc.tr.g;
 
-  const int xlo = max(cx0 - pc.r, 0), xhi = min(cx1 + pc.r, pc.nx - 1);
-  for (int dz = -pc.r; dz <= pc.r; dz++) {
-    const int zz = cz + dz;
-    if (zz < 0 || zz >= pc.nz) continue;
-    for (int dy = -pc.r; dy <= pc.r; dy++) {
-      const int yy = cy + dy;
-      if (yy < 0 || yy >= pc.ny) continue;
-      const size_t rbase = ((size_t)zz * pc.ny + yy) * pc.nx;
-      const uint32_t lo = pc.cs[rbase + xlo], hi_ = pc.cs[rbase + xhi + 1];
-      for (uint32_t base = lo; base < hi_; base += kWave) {
+  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
+    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
+      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
+      const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+      const uint32_t lo = pc.tr.cs[cell], hi_ = pc.tr.cs[cell + 1];
+      uint32_t tl = pc.tr.toff[cell];
+      for (uint32_t base = lo; base < hi_; base += kWave, tl++) {
+        if (!tile_meets(box, pc.tr.tbox, tl)) continue;
         const uint32_t j = base + lane;
         if (j < hi_) {
           sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
@@ -258,7 +256,6 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
         __syncthreads();
       }
     }
-  }
   if (active) P::finish(a, t, i);
 }
 
@@ -549,25 +546,26 @@ struct RhsAdj {
 };
 
 // cells holding at least one target (the benign race writes the same value)
-__global__ void k_cell_has_target(uint32_t n, const int32_t *bc, const uint32_t *ckey, uint32_t *tcell) {
+__global__ void k_cell_has_target(uint32_t n, const int32_t *bc, const uint32_t *ckey, int sh, uint32_t *tcell) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && need_neigh(bc[i])) tcell[ckey[i]] = 1u;
+  if (i < n && need_neigh(bc[i])) tcell[ckey[i] >> sh] = 1u;
 }
 // Adjoint rows: real particles, erupt ghosts, and boundary ghosts (bc 1, 2:
 // constant in time, but their velocity and state enter their neighbours'
 // sums) whose cell neighbourhood holds a target.
-__global__ void k_adj_row_flags(uint32_t n, const int32_t *bc, const uint32_t *ckey, const uint32_t *tcell,
-                                int nx, int ny, int nz, uint32_t *flags) {
+// rr: cells within the pass reach
+__global__ void k_adj_row_flags(uint32_t n, const int32_t *bc, const uint32_t *ckey, int sh, const uint32_t *tcell,
+                                int nx, int ny, int nz, int rr, uint32_t *flags) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   const int b = i < n ? bc[i] : -1;
   uint32_t f = (is_real(b) || b == 0) ? 1u : 0u;
   if (b == 1 || b == 2) {
-    const uint32_t k = ckey[i];
+    const uint32_t k = ckey[i] >> sh;
     const int cx = (int)(k % (uint32_t)nx), cy = (int)((k / (uint32_t)nx) % (uint32_t)ny), cz = (int)(k / ((uint32_t)nx * ny));
-    for (int dz = -1; dz <= 1 && !f; dz++)
-      for (int dy = -1; dy <= 1 && !f; dy++)
-        for (int dx = -1; dx <= 1 && !f; dx++) {
+    for (int dz = -rr; dz <= rr && !f; dz++)
+      for (int dy = -rr; dy <= rr && !f; dy++)
+        for (int dx = -rr; dx <= rr && !f; dx++) {
           const int x = cx + dx, y = cy + dy, z = cz + dz;
           if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
           if (tcell[((size_t)z * ny + y) * nx + x]) f = 1u;
@@ -622,8 +620,8 @@ struct Adjoint {
   double *aux[5] = {};        // per-particle seeds of one pass
   double *part = nullptr;     // reduction partials
   // chunks of adjoint rows
-  uint32_t *flags = nullptr, *tpos = nullptr, *rows = nullptr, *row_chunks = nullptr, *row_off = nullptr;
-  uint32_t *cfirst = nullptr, *ccount = nullptr, *crow = nullptr, *nchunks = nullptr;
+  uint32_t *flags = nullptr, *tpos = nullptr, *rows = nullptr, *cchunks = nullptr, *coff = nullptr;
+  uint32_t *cfirst = nullptr, *ccount = nullptr, *nchunks = nullptr;
   uint32_t *tcell = nullptr;
   size_t n_cap = 0, rows_cap = 0, chunk_cap = 0, tcell_cap = 0;
 };
@@ -654,12 +652,13 @@ static void snap_free(Snapshot &s) {
 
 static void rec_free(StepRec &r) {
   for (auto &p : r.d) { if (p) hipFree(p); p = nullptr; }
-  void *ptrs[] = {r.pid, r.refb, r.ckey, r.bc, r.phase, r.cs};
+  void *ptrs[] = {r.pid, r.refb, r.ckey, r.bc, r.phase, r.cs, r.toff, r.tbox};
   for (auto p : ptrs) if (p) hipFree(p);
   r.pid = r.refb = r.ckey = nullptr;
   r.bc = r.phase = nullptr;
-  r.cs = nullptr;
-  r.cap = r.cs_cap = r.n = 0;
+  r.cs = r.toff = nullptr;
+  r.tbox = nullptr;
+  r.cap = r.cs_cap = r.tile_cap = r.n = 0;
 }
 
 static void adjoint_free(psph_ctx *c) {
@@ -671,7 +670,7 @@ static void adjoint_free(psph_ctx *c) {
   for (auto p : A->L) if (p) hipFree(p);
   for (auto p : A->l) if (p) hipFree(p);
   for (auto p : A->aux) if (p) hipFree(p);
-  void *ptrs[] = {A->part, A->flags, A->tpos, A->rows, A->row_chunks, A->row_off, A->cfirst, A->ccount, A->crow,
+  void *ptrs[] = {A->part, A->flags, A->tpos, A->rows, A->cchunks, A->coff, A->cfirst, A->ccount,
                   A->nchunks, A->tcell};
   for (auto p : ptrs) if (p) hipFree(p);
   delete A;
@@ -754,13 +753,22 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
     const size_t ncs = (size_t)c->grid.ncells + 1;
     if (ncs > r.cs_cap) {
       if (r.cs) hipFree(r.cs);
-      r.cs = nullptr;
+      if (r.toff) hipFree(r.toff);
+      r.cs = r.toff = nullptr;
       PSPH_TRY(dmalloc((void **)&r.cs, ncs * 4));
+      PSPH_TRY(dmalloc((void **)&r.toff, ncs * 4));
       r.cs_cap = ncs;
+    }
+    const size_t ntb = n / kWave + (size_t)c->grid.ncells + 1;
+    if (ntb > r.tile_cap) {
+      if (r.tbox) hipFree(r.tbox);
+      r.tbox = nullptr;
+      PSPH_TRY(dmalloc((void **)&r.tbox, ntb * 2 * sizeof(float4)));
+      r.tile_cap = ntb;
     }
     r.n = n;
     r.grid = c->grid;
-    r.r_main = c->r_main;
+    r.reach = c->reach_main;
     r.dt = dt;
     r.time = c->time;
     const double *src[] = {c->f.x, c->f.y, c->f.z, c->f.h, c->f.m, c->f.rho, c->f.p, c->f.c, c->f.T, c->f.cp,
@@ -774,6 +782,8 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
       HIP_TRY(hipMemcpyAsync(r.phase, c->f.phase, n * 4, hipMemcpyDeviceToDevice, st));
     }
     HIP_TRY(hipMemcpyAsync(r.cs, c->cs, ncs * 4, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r.toff, c->toff, ncs * 4, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(r.tbox, c->tbox, ntb * 2 * sizeof(float4), hipMemcpyDeviceToDevice, st));
   } else if (stage == 1) {
     PSPH_TRY(cp(RF_V1X, c->f.vx)); PSPH_TRY(cp(RF_V1Y, c->f.vy)); PSPH_TRY(cp(RF_V1Z, c->f.vz));
     PSPH_TRY(cp(RF_E1, c->f.e));
@@ -825,11 +835,11 @@ static int dev_sum(psph_ctx *c, Adjoint &A, const double *v, size_t n, double *o
   return 0;
 }
 
-// chunks of adjoint rows (real particles and erupt ghosts) of one step
+// chunks of adjoint rows of one step (<= 64 consecutive rows of one cell)
 static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
   const uint32_t n = (uint32_t)R.n;
   const GridDesc &g = R.grid;
-  const uint32_t nrows = (uint32_t)g.ny * g.nz;
+  const uint32_t ncells = g.ncells;
   hipStream_t s = c->stream;
   if (n + 1 > A.n_cap) {
     void *ptrs[] = {A.flags, A.tpos, A.rows};
@@ -840,54 +850,42 @@ static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
     PSPH_TRY(dmalloc((void **)&A.rows, cap * 4));
     A.n_cap = cap;
   }
-  if (nrows + 1 > A.rows_cap) {
-    if (A.row_chunks) hipFree(A.row_chunks);
-    if (A.row_off) hipFree(A.row_off);
-    PSPH_TRY(dmalloc((void **)&A.row_chunks, (nrows + 1) * 4));
-    PSPH_TRY(dmalloc((void **)&A.row_off, (nrows + 1) * 4));
-    A.rows_cap = nrows + 1;
+  if (ncells + 1 > A.tcell_cap) {
+    void *ptrs[] = {A.cchunks, A.coff, A.tcell};
+    for (auto p : ptrs) if (p) hipFree(p);
+    PSPH_TRY(dmalloc((void **)&A.cchunks, ((size_t)ncells + 1) * 4));
+    PSPH_TRY(dmalloc((void **)&A.coff, ((size_t)ncells + 1) * 4));
+    PSPH_TRY(dmalloc((void **)&A.tcell, ((size_t)ncells + 1) * 4));
+    A.tcell_cap = (size_t)ncells + 1;
   }
-  const size_t chunks_ub = n / kWave + nrows + 1;
+  const size_t chunks_ub = n / kWave + (size_t)ncells + 1;
   if (chunks_ub > A.chunk_cap) {
-    void *ptrs[] = {A.cfirst, A.ccount, A.crow};
+    void *ptrs[] = {A.cfirst, A.ccount};
     for (auto p : ptrs) if (p) hipFree(p);
     PSPH_TRY(dmalloc((void **)&A.cfirst, chunks_ub * 4));
     PSPH_TRY(dmalloc((void **)&A.ccount, chunks_ub * 4));
-    PSPH_TRY(dmalloc((void **)&A.crow, chunks_ub * 4));
     A.chunk_cap = chunks_ub;
   }
   if (!A.nchunks) PSPH_TRY(dmalloc((void **)&A.nchunks, 4));
-  const size_t ncells = (size_t)g.nx * g.ny * g.nz;
-  if (ncells > A.tcell_cap) {
-    if (A.tcell) hipFree(A.tcell);
-    A.tcell = nullptr;
-    PSPH_TRY(dmalloc((void **)&A.tcell, ncells * 4));
-    A.tcell_cap = ncells;
-  }
+  const int sh = 3 * g.fbits;
+  const int rr = g.inv_e > 0 ? (int)std::ceil(R.reach * g.inv_e) : 0;
   hipLaunchKernelGGL(k_fill_i32, blocks(ncells), 256, 0, s, (int *)A.tcell, 0, (int)ncells);
-  if (n) hipLaunchKernelGGL(k_cell_has_target, blocks(n), 256, 0, s, n, R.bc, R.ckey, A.tcell);
-  hipLaunchKernelGGL(k_adj_row_flags, blocks(n + 1), 256, 0, s, n, R.bc, R.ckey, A.tcell, g.nx, g.ny, g.nz, A.flags);
-  size_t bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, A.flags, A.tpos, (int)n + 1, s));
-  PSPH_TRY(ensure_cub(c, bytes));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, A.flags, A.tpos, (int)n + 1, s));
+  if (n) hipLaunchKernelGGL(k_cell_has_target, blocks(n), 256, 0, s, n, R.bc, R.ckey, sh, A.tcell);
+  hipLaunchKernelGGL(k_adj_row_flags, blocks(n + 1), 256, 0, s, n, R.bc, R.ckey, sh, A.tcell, g.nx, g.ny, g.nz,
+                     rr, A.flags);
+  PSPH_TRY(excl_scan(c, A.flags, A.tpos, (size_t)n + 1));
   if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, A.flags, A.tpos, A.rows);
-  hipLaunchKernelGGL(k_row_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, R.cs, A.tpos, A.row_chunks);
-  bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, A.row_chunks, A.row_off, (int)nrows + 1, s));
-  PSPH_TRY(ensure_cub(c, bytes));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, A.row_chunks, A.row_off, (int)nrows + 1, s));
-  hipLaunchKernelGGL(k_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, R.cs, A.tpos, A.row_off,
-                     A.cfirst, A.ccount, A.crow, A.nchunks);
+  hipLaunchKernelGGL(k_cell_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, A.cchunks);
+  PSPH_TRY(excl_scan(c, A.cchunks, A.coff, (size_t)ncells + 1));
+  hipLaunchKernelGGL(k_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, A.coff,
+                     A.cfirst, A.ccount, A.nchunks);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
 template <class P>
 static void launch_adj(psph_ctx *c, const AdjCommon &pc, const StepRec &R, const typename P::Args &a) {
-  size_t ub = R.n / kWave + (size_t)R.grid.ny * R.grid.nz + 1 + kXcds;
-  unsigned nb = (unsigned)((ub + kXcds - 1) / kXcds * kXcds);
-  hipLaunchKernelGGL(k_adj_pairs<P>, nb, kWave, 0, c->stream, pc, a);
+  hipLaunchKernelGGL(k_adj_pairs<P>, chunk_grid(R.n, R.grid.ncells), kWave, 0, c->stream, pc, a);
 }
 
 struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };
@@ -947,10 +945,10 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
   PSPH_TRY(adj_rows(c, A, R));
   AdjCommon pc;
   pc.x = R.d[RF_X]; pc.y = R.d[RF_Y]; pc.z = R.d[RF_Z]; pc.h = R.d[RF_H]; pc.h_list = R.d[RF_HLIST];
-  pc.refb = R.refb; pc.cellkey = R.ckey; pc.cs = R.cs; pc.rows = A.rows;
-  pc.cfirst = A.cfirst; pc.ccount = A.ccount; pc.crow = A.crow; pc.nchunks = A.nchunks;
+  pc.refb = R.refb; pc.rows = A.rows;
+  pc.cfirst = A.cfirst; pc.ccount = A.ccount; pc.nchunks = A.nchunks;
   pc.bc = R.bc;
-  pc.nx = R.grid.nx; pc.ny = R.grid.ny; pc.nz = R.grid.nz; pc.r = R.r_main;
+  pc.tr = Traversal{R.cs, R.toff, R.tbox, R.grid, R.reach};
   pc.list_all = k.list_all ? 1 : 0;
   pc.list_factor = k.list_factor; pc.cutoff2 = k.cutoff2;
   pc.cut = cut_test(k.cutoff);

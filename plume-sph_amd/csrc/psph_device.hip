@@ -137,25 +137,44 @@ __global__ void k_h_max(uint32_t n, const int32_t *bc, const double *h, unsigned
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 
+// Cell grid of the neighbour search.  Particles are sorted by
+// (cell, Morton code of their position inside the cell at fbits bits per
+// axis), so every cell is a contiguous index range and every run of 64
+// consecutive particles of a cell (a "tile") is a compact cluster.
 struct GridDesc {
   double lo[3];
-  double inv_e;
+  double inv_e;      // 1 / cell edge; 0: one cell holds everything
   int nx, ny, nz;
   uint32_t ncells;
+  int fbits;
 };
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// floor((v - lo) * inv_e) clamped to [0, n - 1] without overflowing the int conversion
+__device__ inline int cell_of(double v, double lo, double inv_e, int n) {
+  const double f = floor((v - lo) * inv_e);
+  return (int)fmin(fmax(f, 0.0), (double)(n - 1));
+}
 
 __global__ void k_cellkey(uint32_t n, const double *x, const double *y, const double *z,
                           const int32_t *bc, GridDesc g, uint32_t *key, uint32_t *idx) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   idx[i] = i;
-  if (bc[i] < 0) { key[i] = g.ncells; return; }   // removed: sorts past every cell
-  int cx = clampi((int)floor((x[i] - g.lo[0]) * g.inv_e), 0, g.nx - 1);
-  int cy = clampi((int)floor((y[i] - g.lo[1]) * g.inv_e), 0, g.ny - 1);
-  int cz = clampi((int)floor((z[i] - g.lo[2]) * g.inv_e), 0, g.nz - 1);
-  key[i] = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+  const int sh = 3 * g.fbits;
+  if (bc[i] < 0) { key[i] = g.ncells << sh; return; }   // removed: sorts past every cell
+  const double p[3] = {x[i], y[i], z[i]};
+  const int nn[3] = {g.nx, g.ny, g.nz};
+  int cc[3];
+  uint32_t m = 0;
+  const double nf = (double)(1 << g.fbits);
+  for (int d = 0; d < 3; d++) {
+    const double f = (p[d] - g.lo[d]) * g.inv_e;
+    cc[d] = (int)fmin(fmax(floor(f), 0.0), (double)(nn[d] - 1));
+    const int u = (int)fmin(fmax(floor((f - cc[d]) * nf), 0.0), nf - 1);
+    for (int b = 0; b < g.fbits; b++) m |= (uint32_t)((u >> b) & 1) << (3 * b + d);
+  }
+  key[i] = ((((uint32_t)cc[2] * g.ny + cc[1]) * g.nx + cc[0]) << sh) | m;
 }
 
 template <class T>
@@ -164,13 +183,44 @@ __global__ void k_gather(uint32_t n, const uint32_t *perm, const T *__restrict__
   if (i < n) dst[i] = src[perm[i]];
 }
 
-// cell_start[c] = first sorted index whose key >= c, for c in [0, ncells]
-__global__ void k_cell_start(uint32_t n, const uint32_t *key, uint32_t ncells, uint32_t *cs) {
+// cell_start[c] = first sorted index whose cell (key >> sh) >= c, c in [0, ncells]
+__global__ void k_cell_start(uint32_t n, const uint32_t *key, int sh, uint32_t ncells, uint32_t *cs) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
-  uint32_t prev = (i == 0) ? 0u : min(key[i - 1], ncells) + 1u;
-  uint32_t cur = (i == n) ? ncells + 1u : min(key[i], ncells) + 1u;
+  uint32_t prev = (i == 0) ? 0u : min(key[i - 1] >> sh, ncells) + 1u;
+  uint32_t cur = (i == n) ? ncells + 1u : min(key[i] >> sh, ncells) + 1u;
   for (uint32_t c = prev; c < cur; c++) cs[c] = i;
+}
+
+// tiles per cell (64 consecutive particles each); tcnt[ncells] = 0
+__global__ void k_tile_counts(uint32_t ncells, const uint32_t *cs, uint32_t *tcnt) {
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncells) return;
+  tcnt[c] = c == ncells ? 0u : (cs[c + 1] - cs[c] + kWave - 1) / kWave;
+}
+
+// Bounding box of every tile, rounded outwards to float: [lo, hi] per tile.
+// One wavefront per tile; the tile's cell is found by bisection of toff.
+__global__ __launch_bounds__(kWave) void k_tile_boxes(uint32_t ncells, const uint32_t *cs, const uint32_t *toff,
+                                                      const double *x, const double *y, const double *z,
+                                                      float4 *tbox) {
+  const uint32_t t = blockIdx.x;
+  if (t >= toff[ncells]) return;
+  uint32_t lo = 0, hi = ncells;              // toff[lo] <= t < toff[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (toff[mid] <= t) lo = mid; else hi = mid;
+  }
+  const uint32_t base = cs[lo] + kWave * (t - toff[lo]);
+  const uint32_t end = min(cs[lo + 1], base + kWave);
+  const uint32_t j = base + threadIdx.x < end ? base + threadIdx.x : base;
+  const double px = x[j], py = y[j], pz = z[j];
+  const double ax = wave_min(px), ay = wave_min(py), az = wave_min(pz);
+  const double bx = wave_max(px), by = wave_max(py), bz = wave_max(pz);
+  if (threadIdx.x == 0) {
+    tbox[2 * t] = make_float4(__double2float_rd(ax), __double2float_rd(ay), __double2float_rd(az), 0.f);
+    tbox[2 * t + 1] = make_float4(__double2float_ru(bx), __double2float_ru(by), __double2float_ru(bz), 0.f);
+  }
 }
 
 // reference bucket id (preprocessor cells of edge 8 h_air, preprocess.cc:88-133)
@@ -198,29 +248,27 @@ __global__ void k_targets(uint32_t n, const uint32_t *flags, const uint32_t *tpo
   if (i < n && flags[i]) targets[tpos[i]] = i;
 }
 
-__global__ void k_row_chunks(uint32_t nrows, int nx, const uint32_t *cs, const uint32_t *tpos,
-                             uint32_t *row_chunks) {
+// chunks: up to 64 consecutive targets of one cell
+__global__ void k_cell_chunks(uint32_t ncells, const uint32_t *cs, const uint32_t *tpos, uint32_t *cell_chunks) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r > nrows) return;
-  if (r == nrows) { row_chunks[r] = 0; return; }
-  uint32_t a = tpos[cs[(size_t)r * nx]], b = tpos[cs[(size_t)(r + 1) * nx]];
-  row_chunks[r] = (b - a + kWave - 1) / kWave;
+  if (r > ncells) return;
+  if (r == ncells) { cell_chunks[r] = 0; return; }
+  uint32_t a = tpos[cs[r]], b = tpos[cs[r + 1]];
+  cell_chunks[r] = (b - a + kWave - 1) / kWave;
 }
 
-__global__ void k_chunks(uint32_t nrows, int nx, const uint32_t *cs, const uint32_t *tpos,
-                         const uint32_t *row_off, uint32_t *cfirst, uint32_t *ccount,
-                         uint32_t *crow, uint32_t *nchunks) {
+__global__ void k_chunks(uint32_t ncells, const uint32_t *cs, const uint32_t *tpos, const uint32_t *chunk_off,
+                         uint32_t *cfirst, uint32_t *ccount, uint32_t *nchunks) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) {
-    if (r == nrows) *nchunks = row_off[nrows];
+  if (r >= ncells) {
+    if (r == ncells) *nchunks = chunk_off[ncells];
     return;
   }
-  uint32_t a = tpos[cs[(size_t)r * nx]], b = tpos[cs[(size_t)(r + 1) * nx]];
-  uint32_t o = row_off[r];
+  uint32_t a = tpos[cs[r]], b = tpos[cs[r + 1]];
+  uint32_t o = chunk_off[r];
   for (uint32_t t = a; t < b; t += kWave) {
     cfirst[o] = t;
     ccount[o] = min((uint32_t)kWave, b - t);
-    crow[o] = r;
     o++;
   }
 }
@@ -228,16 +276,52 @@ __global__ void k_chunks(uint32_t nrows, int nx, const uint32_t *cs, const uint3
 // ---------------------------------------------------------------------------
 // Pair-pass skeleton
 // ---------------------------------------------------------------------------
+// Cells and tiles a wavefront's chunk of particles must visit: the chunk's
+// bounding box grown by `reach` (the largest per-axis pair distance any
+// predicate of the pass can accept) selects a block of cells, and of their
+// tiles only those whose box meets the grown box.  Every candidate within
+// reach of a chunk particle lies in such a tile.
+struct Traversal {
+  const uint32_t *cs, *toff;
+  const float4 *tbox;
+  GridDesc g;
+  double reach;
+};
+
+struct ChunkBox {
+  int c0[3], c1[3];
+  float lo[3], hi[3];
+};
+
+__device__ inline ChunkBox chunk_box(const Traversal &tr, const double *p) {
+  ChunkBox b;
+  const int nn[3] = {tr.g.nx, tr.g.ny, tr.g.nz};
+  for (int d = 0; d < 3; d++) {
+    const double lo = wave_min(p[d]) - tr.reach, hi = wave_max(p[d]) + tr.reach;
+    b.lo[d] = __double2float_rd(lo);
+    b.hi[d] = __double2float_ru(hi);
+    if (tr.g.inv_e > 0) {
+      b.c0[d] = cell_of(lo, tr.g.lo[d], tr.g.inv_e, nn[d]);
+      b.c1[d] = cell_of(hi, tr.g.lo[d], tr.g.inv_e, nn[d]);
+    } else {
+      b.c0[d] = b.c1[d] = 0;
+    }
+  }
+  return b;
+}
+
+__device__ inline bool tile_meets(const ChunkBox &b, const float4 *tbox, uint32_t t) {
+  const float4 lo = tbox[2 * t], hi = tbox[2 * t + 1];
+  return !(lo.x > b.hi[0] || hi.x < b.lo[0] || lo.y > b.hi[1] || hi.y < b.lo[1] || lo.z > b.hi[2] || hi.z < b.lo[2]);
+}
+
 struct PairCommon {
-  const double *x, *y, *z, *h;
-  double edge;                // cell edge (0: one cell, all pairs)
+  const double *x, *y, *z;
   const uint32_t *refb;
   const double *h_list;
-  const uint32_t *cellkey;    // sorted cell keys
-  const uint32_t *cs;         // cell_start[ncells+1]
+  Traversal tr;
   const uint32_t *targets;
-  const uint32_t *cfirst, *ccount, *crow, *nchunks;
-  int nx, ny, nz, r;
+  const uint32_t *cfirst, *ccount, *nchunks;
   int list_all;
   double list_factor;
   CutTest cut;
@@ -248,7 +332,6 @@ struct PairCommon {
 template <int D, class P>
 __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
-  __shared__ float sxf[kWave], syf[kWave], szf[kWave], shf[kWave];
   __shared__ uint32_t srefb[kWave];
   __shared__ typename P::Tile tile;
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
@@ -260,111 +343,69 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   if (local >= per) return;
   const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
   if (chunk >= nch) return;
-  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk], row = pc.crow[chunk];
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
   const int lane = threadIdx.x;
   const bool active = (uint32_t)lane < cc;
   const uint32_t i = pc.targets[cf + (active ? lane : 0)];
-  const uint32_t t0 = pc.targets[cf];
-  const int cx0 = (int)(pc.cellkey[t0] % (uint32_t)pc.nx);
-  const int cx1 = (int)(pc.cellkey[pc.targets[cf + cc - 1]] % (uint32_t)pc.nx);
-  const int cy = (int)(row % (uint32_t)pc.ny), cz = (int)(row / (uint32_t)pc.ny);
 
   double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
   const uint32_t rb_i = pc.refb[i];
   const double lf_i = pc.list_factor * pc.h_list[i];
   typename P::Tgt t;
   P::load_target(a, i, t);
+  const ChunkBox box = chunk_box(pc.tr, xi);
+  const GridDesc &g = pc.tr.g;
 
-  // Phase 1 runs in FP32 on coordinates relative to the chunk's first target
-  // (3D cell grid).  Every staged candidate lies within dlim of that origin
-  // (else its tile takes the exact path), so |error of max_k |dx_k|| and the
-  // rounding of the FP32 thresholds stay below m = dlim 2^-19; a comparison
-  // decided by more than m is exact, and the rest re-run the FP64 test.
-  const int xlo = max(cx0 - pc.r, 0), xhi = min(cx1 + pc.r, pc.nx - 1);
-  const double dlim = (double)(max(xhi - xlo + 1, 2 * pc.r + 1) + 1) * pc.edge;
-  const double ox = pc.x[t0], oy = pc.y[t0], oz = pc.z[t0];
-  const bool tfar = active && !(fmax(fabs(xi[0] - ox), fmax(fabs(xi[1] - oy), fabs(xi[2] - oz))) < dlim);
-  const bool f32 = D == 3 && pc.edge > 0 && !__any(tfar);
-  const float m = (float)(dlim * 0x1p-19);
-  const float cutf = (float)pc.cut.cut;
-  const float xif[3] = {(float)(xi[0] - ox), (float)(xi[1] - oy), (float)(xi[2] - oz)};
-  const float lff = (float)lf_i;
   for (int pass = 0; pass < P::npasses(a); pass++) {
     P::begin_pass(a, t, pass);
-    const float sup_lo = (float)t.supp - m, sup_hi = (float)t.supp + m;
-    for (int dz = -pc.r; dz <= pc.r; dz++) {
-      const int zz = cz + dz;
-      if (zz < 0 || zz >= pc.nz) continue;
-      for (int dy = -pc.r; dy <= pc.r; dy++) {
-        const int yy = cy + dy;
-        if (yy < 0 || yy >= pc.ny) continue;
-        const size_t rbase = ((size_t)zz * pc.ny + yy) * pc.nx;
-        const uint32_t lo = pc.cs[rbase + xlo], hi = pc.cs[rbase + xhi + 1];
-        for (uint32_t base = lo; base < hi; base += kWave) {
-          const uint32_t j = base + lane;
-          bool far = false;
-          if (j < hi) {
-            const double xj = pc.x[j], yj = pc.y[j], zj = pc.z[j];
-            sx[lane] = xj; sy[lane] = yj; sz[lane] = zj;
-            srefb[lane] = pc.refb[j];
-            if (f32) {
-              const double rx = xj - ox, ry = yj - oy, rz = zj - oz;
-              far = !(fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) < dlim);
-              sxf[lane] = (float)rx; syf[lane] = (float)ry; szf[lane] = (float)rz;
-              shf[lane] = (float)pc.h[j];
+    for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
+      for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
+        for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
+          const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+          const uint32_t lo = pc.tr.cs[cell], hi = pc.tr.cs[cell + 1];
+          uint32_t tl = pc.tr.toff[cell];
+          for (uint32_t base = lo; base < hi; base += kWave, tl++) {
+            if (!tile_meets(box, pc.tr.tbox, tl)) continue;
+            const uint32_t j = base + lane;
+            if (j < hi) {
+              sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+              srefb[lane] = pc.refb[j];
+              P::stage(a, j, tile, lane);
             }
-            P::stage(a, j, tile, lane);
-          }
-          const bool exact_tile = !f32 || __any(far);
-          __syncthreads();
-          const uint32_t cnt = min((uint32_t)kWave, hi - base);
-          if (active && P::want(t)) {
-            // exact FP64 predicate for candidate q
-            auto exact = [&](uint32_t q) -> bool {
-              double amax = fabs(xi[0] - sx[q]);
-              if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
-              // outside the kernel box (exact, see CutTest)
-              if (pc.cut.outside(amax, P::hker(a, t, tile, q))) return false;
-              // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
-              return (pc.list_all || srefb[q] == rb_i || amax <= lf_i) && amax <= t.supp &&
-                     P::extra(a, t, tile, q, base + q, i);
-            };
-            uint64_t mask = 0;
-            if (exact_tile) {
-              for (uint32_t q = 0; q < cnt; q++)
-                if (exact(q)) mask |= 1ull << q;
-            } else {
-              uint64_t amb = 0;
+            __syncthreads();
+            const uint32_t cnt = min((uint32_t)kWave, hi - base);
+            if (active && P::want(t)) {
+              // phase 1: exact FP64 predicate -> per-lane hit mask
+              uint64_t mask = 0;
               for (uint32_t q = 0; q < cnt; q++) {
-                const float am = fmaxf(fabsf(xif[0] - sxf[q]), fmaxf(fabsf(xif[1] - syf[q]), fabsf(xif[2] - szf[q])));
-                const float tb = cutf * P::hker_f(t, shf[q]);
-                const bool same = pc.list_all || srefb[q] == rb_i;
-                const bool ex = P::extra(a, t, tile, q, base + q, i);
-                const bool in = ex && am <= tb - m && (same || am <= lff - m) && am <= sup_lo;
-                const bool out = !ex || am > tb + m || (!same && am > lff + m) || am > sup_hi;
+                double dx[3];
+                dx[0] = xi[0] - sx[q];
+                double amax = fabs(dx[0]);
+                if (D == 3) {
+                  dx[1] = xi[1] - sy[q];
+                  dx[2] = xi[2] - sz[q];
+                  amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
+                }
+                // outside the kernel box (exact, see CutTest)
+                if (pc.cut.outside(amax, P::hker(a, t, tile, q))) continue;
+                // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
+                const bool in = (pc.list_all || srefb[q] == rb_i || amax <= lf_i) &&
+                                amax <= t.supp && P::extra(a, t, tile, q, base + q, i);
                 if (in) mask |= 1ull << q;
-                else if (!out) amb |= 1ull << q;
               }
-              while (amb) {
-                const uint32_t q = (uint32_t)__builtin_ctzll(amb);
-                amb &= amb - 1;
-                if (exact(q)) mask |= 1ull << q;
+              // phase 2: each lane walks its own hits
+              while (mask) {
+                const uint32_t q = (uint32_t)__builtin_ctzll(mask);
+                mask &= mask - 1;
+                double dx[3];
+                dx[0] = xi[0] - sx[q];
+                if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
+                P::pair(a, t, tile, q, base + q, i, dx);
               }
             }
-            // phase 2: each lane walks its own hits
-            while (mask) {
-              const uint32_t q = (uint32_t)__builtin_ctzll(mask);
-              mask &= mask - 1;
-              double dx[3];
-              dx[0] = xi[0] - sx[q];
-              if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
-              P::pair(a, t, tile, q, base + q, i, dx);
-            }
+            __syncthreads();
           }
-          __syncthreads();
         }
-      }
-    }
     P::end_pass(a, t, pass, i, active);
   }
   if (active) P::finish(a, t, i, pc.err);
@@ -425,7 +466,6 @@ struct RhsPass {
   };
   struct Tgt {
     double hi, supp, rv[3], re;
-    float hif;
     R rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3];
     unsigned npairs;
     bool bad;
@@ -436,7 +476,6 @@ struct RhsPass {
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i];
-    t.hif = (float)t.hi;
     t.supp = a.cutoff2 * t.hi;
     const double rho = a.rho[i];
     t.bad = !(rho > 0);
@@ -457,7 +496,6 @@ struct RhsPass {
   }
   // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2); j != i (dW(0) = 0)
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
-  __device__ static float hker_f(const Tgt &t, float hj) { return 0.5f * (t.hif + hj); }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -569,14 +607,13 @@ struct DensPass {
     R m[kWave], rho[kWave], e[kWave];
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
-  struct Tgt { double hi, supp, r1, r2, wn, en; float hif; };
+  struct Tgt { double hi, supp, r1, r2, wn, en; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi;
-    t.hif = (float)t.hi;
     t.r1 = t.r2 = t.wn = t.en = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
@@ -587,7 +624,6 @@ struct DensPass {
   // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
   // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
-  __device__ static float hker_f(const Tgt &t, float hj) { return 0.5f * (t.hif + hj); }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -684,7 +720,6 @@ struct VsmPass {
     s.ok[l] = contr_vel(a.bc[j]);
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
-  __device__ static float hker_f(const Tgt &t, float) { return (float)t.hi; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -733,11 +768,10 @@ struct AdaptPass {
     R m[kWave];
     int ok[kWave];
   };
-  struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; float Hsf; bool done; };
+  struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; bool done; };
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &a, Tgt &t, int) {
     t.Hs = t.hcur * a.c_smooth;
-    t.Hsf = (float)t.Hs;
     const double iHs = 1.0 / t.Hs;
     t.iHs = (R)iHs;
     t.supp = a.cutoff2 * t.Hs;
@@ -754,7 +788,6 @@ struct AdaptPass {
     s.ok[l] = contr_dens(a.pghost, a.bc[j]);
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.Hs; }
-  __device__ static float hker_f(const Tgt &t, float) { return t.Hsf; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
@@ -807,7 +840,6 @@ struct ListPass {
   __device__ static void stage(const Args &, uint32_t, Tile &, int) {}
   __device__ static double bound(const Args &, const Tgt &, const Tile &, uint32_t) { return HUGE_VAL; }
   __device__ static double hker(const Args &, const Tgt &, const Tile &, uint32_t) { return HUGE_VAL; }
-  __device__ static float hker_f(const Tgt &, float) { return HUGE_VALF; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t, uint32_t) { return true; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &, uint32_t, uint32_t j, uint32_t,
@@ -975,10 +1007,11 @@ struct StepRec {
   double *d[RF_N] = {};
   uint32_t *pid = nullptr, *refb = nullptr, *ckey = nullptr;
   int32_t *bc = nullptr, *phase = nullptr;
-  uint32_t *cs = nullptr;
-  size_t cs_cap = 0;
+  uint32_t *cs = nullptr, *toff = nullptr;   // cell starts and first tile of each cell
+  float4 *tbox = nullptr;                     // tile boxes
+  size_t cs_cap = 0, tile_cap = 0;
   GridDesc grid{};
-  int r_main = 1;
+  double reach = 0;
   double dt = 0, time = 0;
   int erupting = 0;
   uint32_t inj_lo = 0, inj_hi = 0;          // ids of the particles this step injected
@@ -997,9 +1030,10 @@ struct psph_ctx {
   double *h_list = nullptr;
   uint32_t *refb = nullptr, *key_a = nullptr, *key_b = nullptr, *idx_a = nullptr, *idx_b = nullptr;
   uint32_t *flags = nullptr, *tpos = nullptr, *targets = nullptr;
-  uint32_t *cfirst = nullptr, *ccount = nullptr, *crow = nullptr;
-  uint32_t *cs = nullptr, *row_chunks = nullptr, *row_off = nullptr, *nchunks = nullptr;
-  size_t cs_cap = 0, rows_cap = 0, chunk_cap = 0;
+  uint32_t *cfirst = nullptr, *ccount = nullptr;
+  uint32_t *cs = nullptr, *tcnt = nullptr, *toff = nullptr, *cchunks = nullptr, *coff = nullptr, *nchunks = nullptr;
+  float4 *tbox = nullptr;
+  size_t cs_cap = 0, tile_cap = 0, chunk_cap = 0;
   void *cub = nullptr;
   size_t cub_bytes = 0;
   unsigned long long *d_u64 = nullptr;   // [0] dt bits [1] hmax bits [2] pairs [3] dead
@@ -1008,7 +1042,7 @@ struct psph_ctx {
   // grid of the last neighbour search
   bool grid_valid = false;
   GridDesc grid{};
-  int r_main = 1;
+  double reach_main = 0;       // per-axis reach of the RHS / density / filter predicates
   double h_max = 0;
   // clock (TimeProps)
   double time = 0, next_output = 0;
@@ -1143,8 +1177,8 @@ extern "C" int psph_destroy(psph_ctx *c) {
 #undef F_
   for (auto p : c->dsp) if (p) hipFree(p);
   void *ptrs[] = {c->usp, c->isp, c->h_list, c->pid, c->pid_sp, c->refb, c->key_a, c->key_b, c->idx_a, c->idx_b,
-                  c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->crow, c->cs,
-                  c->row_chunks, c->row_off, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part};
+                  c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->cs, c->tcnt, c->toff,
+                  c->cchunks, c->coff, c->tbox, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part};
   for (auto p : ptrs) if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -1274,16 +1308,20 @@ static int build_grid(psph_ctx *c) {
   std::memcpy(&hmax, &bits, 8);
   if (!(hmax > 0) || !std::isfinite(hmax)) hmax = k.h_air;
   c->h_max = hmax;
+  c->reach_main = k.cutoff * hmax * (1 + 1e-9);
   GridDesc g{};
   if (k.list_all || k.dim == 1) {
     g.lo[0] = g.lo[1] = g.lo[2] = 0;
     g.inv_e = 0;
     g.nx = g.ny = g.nz = 1;
-    c->r_main = 0;
   } else {
-    // cell edge covers every kernel support of the RHS / density / velocity
-    // passes: |dx_k| <= 3 (h_i + h_j) / 2 <= 3 h_max
-    double E = k.cutoff * hmax * (1 + 1e-9);
+    // cell edge: at least the reach of the RHS / density / velocity passes
+    // (|dx_k| <= 3 (h_i + h_j) / 2 <= 3 h_max) and about 125 particles per
+    // cell, so that a cell holds whole 64-particle chunks
+    double vol = 1;
+    for (int d = 0; d < 3; d++) vol *= k.ext_hi[d] - k.ext_lo[d];
+    const double spacing = std::cbrt(vol / std::max<double>((double)c->n, 1.0));
+    const double E = std::max(c->reach_main, 5.0 * spacing);
     int n3[3];
     for (int d = 0; d < 3; d++) {
       g.lo[d] = k.ext_lo[d] - E;
@@ -1294,37 +1332,43 @@ static int build_grid(psph_ctx *c) {
     if (ncell > 2.0e8) return fail(PSPH_ECAPACITY, "cell grid of %.3g cells (h_max %.4g) too large", ncell, hmax);
     g.inv_e = 1.0 / E;
     g.nx = n3[0]; g.ny = n3[1]; g.nz = n3[2];
-    c->r_main = 1;
   }
   g.ncells = (uint32_t)g.nx * g.ny * g.nz;
+  int cbits = 0;
+  while (cbits < 32 && ((uint64_t)1 << cbits) <= (uint64_t)g.ncells) cbits++;
+  g.fbits = g.inv_e > 0 ? std::min(4, (32 - cbits) / 3) : 0;
   c->grid = g;
   size_t need = (size_t)g.ncells + 1;
   if (need > c->cs_cap) {
-    if (c->cs) hipFree(c->cs);
-    c->cs = nullptr;
-    PSPH_TRY(dmalloc((void **)&c->cs, need * 4));
+    uint32_t **arrs[] = {&c->cs, &c->tcnt, &c->toff, &c->cchunks, &c->coff};
+    for (auto pp : arrs) { if (*pp) hipFree(*pp); *pp = nullptr; }
+    for (auto pp : arrs) PSPH_TRY(dmalloc((void **)pp, need * 4));
     c->cs_cap = need;
   }
-  size_t nrows = (size_t)g.ny * g.nz;
-  if (nrows + 1 > c->rows_cap) {
-    if (c->row_chunks) hipFree(c->row_chunks);
-    if (c->row_off) hipFree(c->row_off);
-    c->row_chunks = c->row_off = nullptr;
-    PSPH_TRY(dmalloc((void **)&c->row_chunks, (nrows + 1) * 4));
-    PSPH_TRY(dmalloc((void **)&c->row_off, (nrows + 1) * 4));
-    c->rows_cap = nrows + 1;
+  const size_t ub = c->n / kWave + (size_t)g.ncells + 1;
+  if (ub > c->tile_cap) {
+    if (c->tbox) hipFree(c->tbox);
+    c->tbox = nullptr;
+    PSPH_TRY(dmalloc((void **)&c->tbox, ub * 2 * sizeof(float4)));
+    c->tile_cap = ub;
   }
-  size_t chunks_ub = c->n / kWave + nrows + 1;
-  if (chunks_ub > c->chunk_cap) {
+  if (ub > c->chunk_cap) {
     if (c->cfirst) hipFree(c->cfirst);
     if (c->ccount) hipFree(c->ccount);
-    if (c->crow) hipFree(c->crow);
-    c->cfirst = c->ccount = c->crow = nullptr;
-    PSPH_TRY(dmalloc((void **)&c->cfirst, chunks_ub * 4));
-    PSPH_TRY(dmalloc((void **)&c->ccount, chunks_ub * 4));
-    PSPH_TRY(dmalloc((void **)&c->crow, chunks_ub * 4));
-    c->chunk_cap = chunks_ub;
+    c->cfirst = c->ccount = nullptr;
+    PSPH_TRY(dmalloc((void **)&c->cfirst, ub * 4));
+    PSPH_TRY(dmalloc((void **)&c->ccount, ub * 4));
+    c->chunk_cap = ub;
   }
+  return 0;
+}
+
+// exclusive scan of n + 1 values (the last input is 0) with hipcub
+static int excl_scan(psph_ctx *c, const uint32_t *in, uint32_t *out, size_t n1) {
+  size_t bytes = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n1, c->stream));
+  PSPH_TRY(ensure_cub(c, bytes));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, in, out, (int)n1, c->stream));
   return 0;
 }
 
@@ -1338,7 +1382,7 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   if (n) {
     hipLaunchKernelGGL(k_cellkey, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.bc, g, c->key_a, c->idx_a);
     int end_bit = 1;
-    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= (uint64_t)g.ncells) end_bit++;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= ((uint64_t)g.ncells << (3 * g.fbits))) end_bit++;
     size_t bytes = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, c->key_a, c->key_b, c->idx_a, c->idx_b, (int)n, 0, end_bit, s));
     PSPH_TRY(ensure_cub(c, bytes));
@@ -1367,7 +1411,12 @@ extern "C" int psph_neighbors(psph_ctx *c) {
     c->n = n;
     hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, s, c->d_u64 + 3, 0ull, 1);
   }
-  hipLaunchKernelGGL(k_cell_start, blocks(n + 1), 256, 0, s, n, c->key_a, g.ncells, c->cs);
+  hipLaunchKernelGGL(k_cell_start, blocks(n + 1), 256, 0, s, n, c->key_a, 3 * g.fbits, g.ncells, c->cs);
+  // tiles of each cell and their boxes
+  hipLaunchKernelGGL(k_tile_counts, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tcnt);
+  PSPH_TRY(excl_scan(c, c->tcnt, c->toff, (size_t)g.ncells + 1));
+  hipLaunchKernelGGL(k_tile_boxes, (unsigned)(n / kWave + g.ncells + 1), kWave, 0, s, g.ncells, c->cs, c->toff,
+                     c->f.x, c->f.y, c->f.z, c->tbox);
   // reference buckets (list membership) and h at list time
   int nb[3] = {1, 1, 1};
   double del = c->cfg.bucket_edge > 0 ? c->cfg.bucket_edge : 1.0;
@@ -1376,21 +1425,14 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   if (n) hipLaunchKernelGGL(k_refb, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.h,
                             c->cfg.ext_lo[0], c->cfg.ext_lo[1], c->cfg.ext_lo[2], del, nb[0], nb[1], nb[2],
                             c->refb, c->h_list);
-  // targets and chunks
+  // targets and their chunks (<= 64 consecutive targets of one cell)
   hipLaunchKernelGGL(k_target_flags, blocks(n + 1), 256, 0, s, n, c->f.bc, c->flags);
-  size_t bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->flags, c->tpos, (int)n + 1, s));
-  PSPH_TRY(ensure_cub(c, bytes));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, c->flags, c->tpos, (int)n + 1, s));
+  PSPH_TRY(excl_scan(c, c->flags, c->tpos, (size_t)n + 1));
   if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, c->flags, c->tpos, c->targets);
-  uint32_t nrows = (uint32_t)g.ny * g.nz;
-  hipLaunchKernelGGL(k_row_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, c->cs, c->tpos, c->row_chunks);
-  bytes = 0;
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c->row_chunks, c->row_off, (int)nrows + 1, s));
-  PSPH_TRY(ensure_cub(c, bytes));
-  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->cub, bytes, c->row_chunks, c->row_off, (int)nrows + 1, s));
-  hipLaunchKernelGGL(k_chunks, blocks(nrows + 1), 256, 0, s, nrows, g.nx, c->cs, c->tpos, c->row_off,
-                     c->cfirst, c->ccount, c->crow, c->nchunks);
+  hipLaunchKernelGGL(k_cell_chunks, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tpos, c->cchunks);
+  PSPH_TRY(excl_scan(c, c->cchunks, c->coff, (size_t)g.ncells + 1));
+  hipLaunchKernelGGL(k_chunks, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tpos, c->coff,
+                     c->cfirst, c->ccount, c->nchunks);
   HIP_TRY(hipGetLastError());
   tm.stop();
   c->grid_valid = true;
@@ -1404,14 +1446,17 @@ static CutTest cut_test(double cut) {
   return CutTest{cut, std::ldexp(1.0, e - 54)};
 }
 
-static PairCommon make_common(psph_ctx *c, int r) {
+static Traversal traversal(psph_ctx *c, double reach) {
+  return Traversal{c->cs, c->toff, c->tbox, c->grid, reach};
+}
+
+static PairCommon make_common(psph_ctx *c, double reach) {
   PairCommon pc;
-  pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z; pc.h = c->f.h;
-  pc.edge = c->grid.inv_e > 0 ? 1.0 / c->grid.inv_e : 0.0;
+  pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z;
   pc.refb = c->refb; pc.h_list = c->h_list;
-  pc.cellkey = c->key_a; pc.cs = c->cs; pc.targets = c->targets;
-  pc.cfirst = c->cfirst; pc.ccount = c->ccount; pc.crow = c->crow; pc.nchunks = c->nchunks;
-  pc.nx = c->grid.nx; pc.ny = c->grid.ny; pc.nz = c->grid.nz; pc.r = r;
+  pc.tr = traversal(c, reach);
+  pc.targets = c->targets;
+  pc.cfirst = c->cfirst; pc.ccount = c->ccount; pc.nchunks = c->nchunks;
   pc.list_all = (c->cfg.list_all || c->cfg.dim == 1) ? 1 : 0;
   pc.list_factor = c->cfg.list_factor;
   pc.cut = cut_test(c->cfg.cutoff);
@@ -1419,16 +1464,16 @@ static PairCommon make_common(psph_ctx *c, int r) {
   return pc;
 }
 
-static unsigned pair_grid(psph_ctx *c) {
-  // upper bound on chunks (every row may add a partial chunk), padded so
-  // that kXcds * ceil(nchunks / kXcds) workgroups always exist
-  size_t ub = c->n / kWave + (size_t)c->grid.ny * c->grid.nz + 1 + kXcds;
+// workgroups for a pair pass: an upper bound on the chunks (each cell may add
+// a partial chunk), padded so that kXcds * ceil(nchunks / kXcds) exist
+static unsigned chunk_grid(size_t n, size_t ncells) {
+  size_t ub = n / kWave + ncells + 1 + kXcds;
   return (unsigned)((ub + kXcds - 1) / kXcds * kXcds);
 }
 
 template <class P>
 static void launch_pairs(psph_ctx *c, const PairCommon &pc, const typename P::Args &a) {
-  unsigned nb = pair_grid(c);
+  unsigned nb = chunk_grid(c->n, c->grid.ncells);
   if (c->cfg.dim == 3)
     hipLaunchKernelGGL((k_pairs<3, Bound<3, P>>), nb, kWave, 0, c->stream, pc, a);
   else
@@ -1456,14 +1501,10 @@ static int reduce_hmax(psph_ctx *c, double *out) {
   return 0;
 }
 
-// the later passes of a step use the new h: widen the cell reach
+// the later passes of a step use the new h: widen the reach
 static void set_reach(psph_ctx *c, double hmax) {
-  const psph_config &k = c->cfg;
   c->h_max = hmax;
-  if (!(k.list_all || k.dim == 1)) {
-    const double E = 1.0 / c->grid.inv_e;
-    c->r_main = std::max(1, (int)std::ceil(k.cutoff * hmax * (1 + 1e-9) / E - 1e-12));
-  }
+  c->reach_main = c->cfg.cutoff * hmax * (1 + 1e-9);
 }
 
 extern "C" int psph_adapt_h(psph_ctx *c) {
@@ -1471,9 +1512,8 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
   if (!c->grid_valid) return fail(PSPH_EINVAL, "adapt_h: call psph_neighbors first");
   const psph_config &k = c->cfg;
   const bool all = k.list_all || k.dim == 1;
-  const double E = all ? 0.0 : 1.0 / c->grid.inv_e;
   // the list never reaches past the own bucket or the 4.25 h_i box of list time
-  const double list_reach = std::max(k.list_factor * c->h_max, k.bucket_edge);
+  const double list_reach = all ? HUGE_VAL : std::max(k.list_factor * c->h_max, k.bucket_edge);
   AdaptArgs a{};
   a.m = c->f.m; a.bc = c->f.bc; a.h = c->f.h; a.done = c->usp;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.c_smooth = k.c_smooth; a.eta = k.eta;
@@ -1481,14 +1521,10 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
   if (c->n) HIP_TRY(hipMemsetAsync(c->usp, 0, c->n * 4, c->stream));
   double hmax = c->h_max;
   for (int it = 0; it < k.num_loop; it++) {
-    int r = 0;
-    if (!all) {
-      double reach = std::min(k.cutoff * k.c_smooth * hmax, list_reach) * (1 + 1e-9);
-      r = std::max(1, (int)std::ceil(reach / E - 1e-12));
-    }
+    const double reach = std::min(k.cutoff * k.c_smooth * hmax, list_reach) * (1 + 1e-9);
     {
       StageTimer tm(c, 2);
-      if (c->n) launch_form<AdaptPass>(c, make_common(c, r), a);
+      if (c->n) launch_form<AdaptPass>(c, make_common(c, reach), a);
     }
     HIP_TRY(hipGetLastError());
     PSPH_TRY(reduce_hmax(c, &hmax));
@@ -1531,7 +1567,7 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 2, 0ull, 1);
   {
     StageTimer tm(c, 3);
-    if (n) launch_form<RhsPass>(c, make_common(c, c->r_main), a);
+    if (n) launch_form<RhsPass>(c, make_common(c, c->reach_main), a);
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
@@ -1560,7 +1596,7 @@ extern "C" int psph_density(psph_ctx *c) {
   a.o_wn = c->rec ? c->rec->d[RF_WNE] : nullptr;
   {
     StageTimer tm(c, 4);
-    if (n) launch_form<DensPass>(c, make_common(c, c->r_main), a);
+    if (n) launch_form<DensPass>(c, make_common(c, c->reach_main), a);
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.rho, c->dsp[1]);
@@ -1580,7 +1616,7 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   a.o_wn = c->rec ? c->rec->d[RF_WNV] : nullptr;
   {
     StageTimer tm(c, 5);
-    if (c->n) launch_form<VsmPass>(c, make_common(c, c->r_main), a);
+    if (c->n) launch_form<VsmPass>(c, make_common(c, c->reach_main), a);
   }
   HIP_TRY(hipGetLastError());
   return check_err(c, "vsmooth");
@@ -1797,12 +1833,9 @@ extern "C" int psph_get_neighbors(psph_ctx *c, int64_t *offsets, size_t n_cap, u
   if (!c->grid_valid) return fail(PSPH_EINVAL, "get_neighbors: call psph_neighbors first");
   const psph_config &k = c->cfg;
   uint32_t n = (uint32_t)c->n;
-  int r = 0;
-  if (!(k.list_all || k.dim == 1)) {
-    double E = 1.0 / c->grid.inv_e;
-    double reach = std::max(k.list_factor * c->h_max, k.bucket_edge);
-    r = std::max(1, (int)std::ceil(reach / E - 1e-12));
-  }
+  // the list rule reaches the own bucket or the 4.25 h_i box (all pairs in 1D)
+  const double r = (k.list_all || k.dim == 1) ? HUGE_VAL
+                                              : std::max(k.list_factor * c->h_max, k.bucket_edge) * (1 + 1e-9);
   uint32_t *cnt = nullptr, *off = nullptr, *out = nullptr;
   PSPH_TRY(dmalloc((void **)&cnt, ((size_t)n + 1) * 4));
   PSPH_TRY(dmalloc((void **)&off, ((size_t)n + 1) * 4));
