@@ -875,17 +875,17 @@ static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
                      rr, A.flags);
   PSPH_TRY(excl_scan(c, A.flags, A.tpos, (size_t)n + 1));
   if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, A.flags, A.tpos, A.rows);
-  hipLaunchKernelGGL(k_cell_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, A.cchunks);
+  hipLaunchKernelGGL(k_cell_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, (uint32_t)kWave, A.cchunks);
   PSPH_TRY(excl_scan(c, A.cchunks, A.coff, (size_t)ncells + 1));
   hipLaunchKernelGGL(k_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, A.coff,
-                     A.cfirst, A.ccount, A.nchunks);
+                     (uint32_t)kWave, A.cfirst, A.ccount, A.nchunks);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
 template <class P>
 static void launch_adj(psph_ctx *c, const AdjCommon &pc, const StepRec &R, const typename P::Args &a) {
-  hipLaunchKernelGGL(k_adj_pairs<P>, chunk_grid(R.n, R.grid.ncells), kWave, 0, c->stream, pc, a);
+  hipLaunchKernelGGL(k_adj_pairs<P>, chunk_grid(R.n, R.grid.ncells, kWave), kWave, 0, c->stream, pc, a);
 }
 
 struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -248,17 +249,18 @@ __global__ void k_targets(uint32_t n, const uint32_t *flags, const uint32_t *tpo
   if (i < n && flags[i]) targets[tpos[i]] = i;
 }
 
-// chunks: up to 64 consecutive targets of one cell
-__global__ void k_cell_chunks(uint32_t ncells, const uint32_t *cs, const uint32_t *tpos, uint32_t *cell_chunks) {
+// chunks: up to csz consecutive targets of one cell
+__global__ void k_cell_chunks(uint32_t ncells, const uint32_t *cs, const uint32_t *tpos, uint32_t csz,
+                              uint32_t *cell_chunks) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r > ncells) return;
   if (r == ncells) { cell_chunks[r] = 0; return; }
   uint32_t a = tpos[cs[r]], b = tpos[cs[r + 1]];
-  cell_chunks[r] = (b - a + kWave - 1) / kWave;
+  cell_chunks[r] = (b - a + csz - 1) / csz;
 }
 
 __global__ void k_chunks(uint32_t ncells, const uint32_t *cs, const uint32_t *tpos, const uint32_t *chunk_off,
-                         uint32_t *cfirst, uint32_t *ccount, uint32_t *nchunks) {
+                         uint32_t csz, uint32_t *cfirst, uint32_t *ccount, uint32_t *nchunks) {
   uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= ncells) {
     if (r == ncells) *nchunks = chunk_off[ncells];
@@ -266,9 +268,9 @@ __global__ void k_chunks(uint32_t ncells, const uint32_t *cs, const uint32_t *tp
   }
   uint32_t a = tpos[cs[r]], b = tpos[cs[r + 1]];
   uint32_t o = chunk_off[r];
-  for (uint32_t t = a; t < b; t += kWave) {
+  for (uint32_t t = a; t < b; t += csz) {
     cfirst[o] = t;
-    ccount[o] = min((uint32_t)kWave, b - t);
+    ccount[o] = min(csz, b - t);
     o++;
   }
 }
@@ -328,7 +330,22 @@ struct PairCommon {
   int *err;
 };
 
-// Each pass supplies: Tile (LDS SoA), Tgt (registers), stage(), pair(), finish().
+// Work split of a wavefront: kChunk targets (consecutive in the cell's fine
+// Morton order, so they sit within ~2 particle spacings of each other), each
+// served by kSub lanes that take every kSub-th candidate of a staged tile.
+// The wave streams the union of its targets' neighbourhoods; with so compact
+// a chunk that union is little more than one neighbourhood, and a lane's hit
+// count per tile is bounded by 64 / kSub, so lanes rarely idle in phase 2.
+// The kSub partial sums of a target are combined by a fixed shuffle tree.
+#ifndef PSPH_SUB_LANES
+#define PSPH_SUB_LANES 2
+#endif
+constexpr int kSub = PSPH_SUB_LANES;
+constexpr double kFar = 1.0e300;
+constexpr int kChunk = kWave / kSub;
+
+// Each pass supplies: Tile (LDS SoA), Tgt (registers), stage(), pair(),
+// combine() (sums a target's sub-lane partials), finish().
 template <int D, class P>
 __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
@@ -345,8 +362,9 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   if (chunk >= nch) return;
   const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
   const int lane = threadIdx.x;
-  const bool active = (uint32_t)lane < cc;
-  const uint32_t i = pc.targets[cf + (active ? lane : 0)];
+  const uint32_t slot = (uint32_t)lane / P::kSub, sub = (uint32_t)lane % P::kSub;
+  const bool active = slot < cc;
+  const uint32_t i = pc.targets[cf + (active ? slot : 0)];
 
   double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
   const uint32_t rb_i = pc.refb[i];
@@ -355,6 +373,8 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   P::load_target(a, i, t);
   const ChunkBox box = chunk_box(pc.tr, xi);
   const GridDesc &g = pc.tr.g;
+  P::init(tile, lane);
+  __syncthreads();
 
   for (int pass = 0; pass < P::npasses(a); pass++) {
     P::begin_pass(a, t, pass);
@@ -371,13 +391,19 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
               sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
               srefb[lane] = pc.refb[j];
               P::stage(a, j, tile, lane);
+            } else {
+              // empty slot: a position no predicate accepts (|dx| > every support)
+              sx[lane] = sy[lane] = sz[lane] = kFar;
             }
             __syncthreads();
             const uint32_t cnt = min((uint32_t)kWave, hi - base);
             if (active && P::want(t)) {
-              // phase 1: exact FP64 predicate -> per-lane hit mask
+              // phase 1: exact FP64 predicate on this lane's share of the tile
+              // (the same trip count on every lane; q < 64 always)
               uint64_t mask = 0;
-              for (uint32_t q = 0; q < cnt; q++) {
+              const uint32_t nb = (cnt + P::kSub - 1) / P::kSub;
+              for (uint32_t b = 0; b < nb; b++) {
+                const uint32_t q = sub + b * P::kSub;
                 double dx[3];
                 dx[0] = xi[0] - sx[q];
                 double amax = fabs(dx[0]);
@@ -391,12 +417,13 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
                 // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
                 const bool in = (pc.list_all || srefb[q] == rb_i || amax <= lf_i) &&
                                 amax <= t.supp && P::extra(a, t, tile, q, base + q, i);
-                if (in) mask |= 1ull << q;
+                if (in) mask |= 1ull << b;
               }
               // phase 2: each lane walks its own hits
               while (mask) {
-                const uint32_t q = (uint32_t)__builtin_ctzll(mask);
+                const uint32_t b = (uint32_t)__builtin_ctzll(mask);
                 mask &= mask - 1;
+                const uint32_t q = sub + b * P::kSub;
                 double dx[3];
                 dx[0] = xi[0] - sx[q];
                 if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
@@ -406,9 +433,22 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
             __syncthreads();
           }
         }
+    if constexpr (P::kSub > 1) P::combine(t);
     P::end_pass(a, t, pass, i, active);
   }
-  if (active) P::finish(a, t, i, pc.err);
+  if (active && sub == 0) P::finish(a, t, i, pc.err);
+}
+
+// sum of v over the kSub lanes of a target (lanes slot*kSub .. +kSub-1)
+template <int S>
+__device__ inline double sub_sum(double v) {
+  for (int o = 1; o < S; o <<= 1) v += __shfl_xor(v, o);
+  return v;
+}
+template <int S>
+__device__ inline uint32_t sub_sum_u(uint32_t v) {
+  for (int o = 1; o < S; o <<= 1) v += (uint32_t)__shfl_xor((int)v, o);
+  return v;
 }
 
 __device__ inline bool in_support(const double *dx, double h, int D) {
@@ -443,6 +483,27 @@ template <> __device__ inline double rrcp<double>(double x) { return 1.0 / x; }
 template <> __device__ inline float rrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <class R> __device__ inline R rabs(R x) { return x < R(0) ? -x : x; }
 
+// exp(-e) in FP64 for 0 <= e < kETab / 16 from a table of exp(-k/16) and the
+// Taylor series of exp(-r), 0 <= r < 1/16, to r^8 (truncation < 4e-17).
+constexpr int kETab = 448;            // covers the kernel box: e = |s|^2 <= 27
+__device__ inline void etab_fill(double *tab, int lane) {
+  for (int k = lane; k < kETab; k += kWave) tab[k] = exp(-0.0625 * k);
+}
+__device__ inline double exp_neg_tab(double e, const double *tab) {
+  const double k = fmin(floor(e * 16.0), (double)(kETab - 1));
+  const double r = e - 0.0625 * k;
+  double p = 1.0 / 40320;
+  p = fma(p, -r, 1.0 / 5040); p = fma(p, -r, 1.0 / 720); p = fma(p, -r, 1.0 / 120);
+  p = fma(p, -r, 1.0 / 24); p = fma(p, -r, 1.0 / 6); p = fma(p, -r, 0.5);
+  p = fma(p, -r, 1.0); p = fma(p, -r, 1.0);
+  return tab[(int)k] * p;
+}
+
+template <class R> __device__ inline R pow16(R x) {
+  x *= x; x *= x; x *= x;
+  return x * x;
+}
+
 // ---------------------------------------------------------------------------
 // RHS pass: mom_engr_update, USE_GSPH==0 (mom_engr_update.cc:30-439)
 // ---------------------------------------------------------------------------
@@ -456,31 +517,52 @@ struct RhsArgs {
   unsigned long long *pairs;
 };
 
+// The FP32-pair form keeps the pressure term m_j (p_j/rho_j^2 + p_i/rho_i^2)
+// grad W in FP64: in air near hydrostatic balance its pair terms cancel to
+// about 1 % of their size, and FP32 rounding there is what the velocities
+// feel.  The kernel value comes from the exp table (FP64 at FP32-like cost);
+// viscosity, heat conduction and the energy work terms stay FP32.
+struct RhsMixTile { double m64[kWave], pv64[kWave], etab[kETab]; };
+struct NoTile {};
+
 template <class R>
 struct RhsPass {
   using Args = RhsArgs;
-  struct Tile {
-    double h[kWave];
+  static constexpr bool kMixed = std::is_same<R, float>::value;
+  struct Tile : std::conditional<kMixed, RhsMixTile, NoTile>::type {
+    double hh[kWave];       // h / 2
     R m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
     R vx[kWave], vy[kWave], vz[kWave];
   };
   struct Tgt {
-    double hi, supp, rv[3], re;
+    double hi, hh, supp, rv[3], re, pv64;
     R rhoi, Vi, pvsqi, ci, Ti, cpi, vel[3];
     unsigned npairs;
-    bool bad;
+    int bad;
   };
+  __device__ static void init(Tile &s, int lane) {
+    if constexpr (kMixed) etab_fill(s.etab, lane);
+  }
+  static constexpr int kSub = ::kSub;
+  __device__ static void combine(Tgt &t) {
+    for (int k = 0; k < 3; k++) t.rv[k] = sub_sum<kSub>(t.rv[k]);
+    t.re = sub_sum<kSub>(t.re);
+    t.npairs = sub_sum_u<kSub>(t.npairs);
+    t.bad = sub_sum_u<kSub>((uint32_t)t.bad) ? 1 : 0;
+  }
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
     t.hi = a.h[i];
+    t.hh = 0.5 * t.hi;
     t.supp = a.cutoff2 * t.hi;
     const double rho = a.rho[i];
-    t.bad = !(rho > 0);
+    t.bad = !(rho > 0) ? 1 : 0;
     const double Vi = 1.0 / rho;
-    t.rhoi = (R)rho; t.Vi = (R)Vi; t.pvsqi = (R)(a.p[i] * Vi * Vi);
+    t.pv64 = a.p[i] * Vi * Vi;
+    t.rhoi = (R)rho; t.Vi = (R)Vi; t.pvsqi = (R)t.pv64;
     t.ci = (R)a.c[i]; t.Ti = (R)a.T[i]; t.cpi = (R)a.cp[i];
     if (a.lans) { t.vel[0] = (R)a.svx[i]; t.vel[1] = (R)a.svy[i]; t.vel[2] = (R)a.svz[i]; }
     else { t.vel[0] = (R)a.vx[i]; t.vel[1] = (R)a.vy[i]; t.vel[2] = (R)a.vz[i]; }
@@ -489,21 +571,27 @@ struct RhsPass {
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
     const double rj = a.rho[j];
     const double V = 1.0 / rj;
-    s.h[l] = a.h[j]; s.m[l] = (R)a.m[j]; s.V[l] = (R)V; s.PV2[l] = (R)(a.p[j] * V * V); s.rho[l] = (R)rj;
+    const double pv = a.p[j] * V * V;
+    s.hh[l] = 0.5 * a.h[j]; s.m[l] = (R)a.m[j]; s.V[l] = (R)V; s.PV2[l] = (R)pv; s.rho[l] = (R)rj;
+    if constexpr (kMixed) { s.m64[l] = a.m[j]; s.pv64[l] = pv; }
     s.c[l] = (R)a.c[j]; s.T[l] = (R)a.T[j]; s.cp[l] = (R)a.cp[j];
     if (a.lans) { s.vx[l] = (R)a.svx[j]; s.vy[l] = (R)a.svy[j]; s.vz[l] = (R)a.svz[j]; }
     else { s.vx[l] = (R)a.vx[j]; s.vy[l] = (R)a.vy[j]; s.vz[l] = (R)a.vz[j]; }
   }
   // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2); j != i (dW(0) = 0)
-  __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
+  __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return t.hh + s.hh[q]; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *dxd) {
     t.npairs++;
     const R rhoj = s.rho[q];
-    if (!(rhoj > R(0))) { t.bad = true; return; }       // assert(uvecj[0] > 0)
-    const R h = (R)(0.5 * (t.hi + s.h[q]));
+    if (!(rhoj > R(0))) { t.bad = 1; return; }          // assert(uvecj[0] > 0)
+    if constexpr (kMixed) {
+      pair_mixed<D>(a, t, s, q, dxd);
+      return;
+    }
+    const R h = (R)(t.hh + s.hh[q]);
     const R ih = rrcp<R>(h);
     R dx[3], si[3], dist_sq = 0, expt = 0;
 #pragma unroll
@@ -563,6 +651,56 @@ struct RhsPass {
     }
     t.re += (double)(R(0.5) * de + heat);
   }
+  template <int D>
+  __device__ static void pair_mixed(const Args &a, Tgt &t, const Tile &s, uint32_t q, const double *dxd) {
+    const double hd = t.hh + s.hh[q];
+    const float h = (float)hd;
+    const float ih = rrcp<float>(h);
+    const double ih64 = (double)ih * fma(-hd, (double)ih, 2.0);     // one Newton step: ~1e-14
+    const double ih2 = ih64 * ih64;
+    double r2 = 0;
+    float dx[3];
+#pragma unroll
+    for (int k = 0; k < D; k++) { r2 = fma(dxd[k], dxd[k], r2); dx[k] = (float)dxd[k]; }
+    const double nk = kInvSqrtPi * ih64;
+    const double norm = (D == 3) ? nk * nk * nk : nk;
+    const double G64 = -2.0 * ih2 * norm * exp_neg_tab(r2 * ih2, s.etab);
+    const float G = (float)G64, dist_sq = (float)r2;
+    const float Vj = s.V[q], rhoj = s.rho[q];
+    float vij[3];
+    vij[0] = t.vel[0] - s.vx[q];
+    if (D == 3) { vij[1] = t.vel[1] - s.vy[q]; vij[2] = t.vel[2] - s.vz[q]; }
+    const float cab = 0.5f * (t.ci + s.c[q]);
+    float vrab = 0, dotvv = 0;
+#pragma unroll
+    for (int k = 0; k < D; k++) { vrab += dx[k] * vij[k]; dotvv += vij[k] * vij[k]; }
+    float vis = 0;
+    if (vrab < 0.f) {
+      const float rhoab = 0.5f * (t.rhoi + rhoj);
+      const float den = dist_sq + (float)a.ata * h * h;
+      const float r = rrcp<float>(den * rhoab);
+      const float miu = h * vrab * (rhoab * r);
+      vis = ((float)a.beta * miu - (float)a.alpha * cab) * miu * (den * r);
+    }
+    if (a.lans) vis -= (float)a.eps_half * dotvv * Vj;
+    // pressure part in FP64, viscous part in FP32; f = (M_p + M_v) G dx
+    const double f = s.m64[q] * (s.pv64[q] + t.pv64) * G64 + (double)(s.m[q] * vis * G);
+#pragma unroll
+    for (int k = 0; k < D; k++) t.rv[k] = fma(-f, dxd[k], t.rv[k]);
+    float heat = 0;
+    if (a.lans && D == 3) {
+      float dotvr = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) dotvr += rabs<float>(vij[k] * dx[k]);
+      const float wr = G * dist_sq;
+      if (dotvr != 0.f && wr != 0.f) {
+        const float cpab = 0.5f * (s.cp[q] + t.cpi);
+        const float kF = (float)(10.0 * a.eps_half) * cpab * dotvv * wr * rrcp<float>((float)a.prandtl * dotvr * h * cab);
+        heat = s.m[q] * Vj * t.Vi * (t.Ti - s.T[q]) * kF;
+      }
+    }
+    t.re += 0.5 * f * (double)vrab + (double)heat;
+  }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (t.bad) { raise_err(err, PSPH_EINVARIANT, i); return; }
     const double gz = -a.g, dt = a.dt;
@@ -597,38 +735,77 @@ struct DensArgs {
   EosConst eos;
   CutTest cut;
   double *o_wn;            // optional: energy-smoothing normalisation (adjoint recompute)
+  int e_sq16;              // e_ratio^2 == 16: exp(-16 x) = exp(-x)^16 by four squarings
 };
 
+struct DensMixTile { double m64[kWave], irho64[kWave], e64[kWave], etab[kETab]; };
+
+// The FP32-pair form evaluates the density and energy-smoothing weights in
+// FP64 with the exp table: the pressure gradient of the next step amplifies
+// relative errors of rho and e by its cancellation.
 template <class R>
 struct DensPass {
   using Args = DensArgs;
-  struct Tile {
-    double h[kWave];
+  static constexpr bool kMixed = std::is_same<R, float>::value;
+  struct Tile : std::conditional<kMixed, DensMixTile, NoTile>::type {
+    double hh[kWave];       // h / 2
     R m[kWave], rho[kWave], e[kWave];
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
-  struct Tgt { double hi, supp, r1, r2, wn, en; };
+  __device__ static void init(Tile &s, int lane) {
+    if constexpr (kMixed) etab_fill(s.etab, lane);
+  }
+  struct Tgt { double hi, hh, supp, r1, r2, wn, en; };
+  static constexpr int kSub = ::kSub;
+  __device__ static void combine(Tgt &t) {
+    t.r1 = sub_sum<kSub>(t.r1); t.r2 = sub_sum<kSub>(t.r2);
+    t.wn = sub_sum<kSub>(t.wn); t.en = sub_sum<kSub>(t.en);
+  }
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
   __device__ static bool want(const Tgt &) { return true; }
   __device__ static void load_target(const Args &a, uint32_t i, Tgt &t) {
-    t.hi = a.h[i]; t.supp = a.cutoff2 * t.hi;
+    t.hi = a.h[i]; t.hh = 0.5 * t.hi; t.supp = a.cutoff2 * t.hi;
     t.r1 = t.r2 = t.wn = t.en = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    s.h[l] = a.h[j]; s.m[l] = (R)a.m[j]; s.rho[l] = (R)a.rho[j]; s.e[l] = (R)a.e[j];
+    s.hh[l] = 0.5 * a.h[j]; s.m[l] = (R)a.m[j]; s.rho[l] = (R)a.rho[j]; s.e[l] = (R)a.e[j];
+    if constexpr (kMixed) { s.m64[l] = a.m[j]; s.irho64[l] = 1.0 / a.rho[j]; s.e64[l] = a.e[j]; }
     int ph = a.phase[j];
     s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
   }
   // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
   // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
-  __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return 0.5 * (t.hi + s.h[q]); }
+  __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return t.hh + s.hh[q]; }
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds) {
-    const double hijd = 0.5 * (t.hi + s.h[q]);
+    const double hijd = t.hh + s.hh[q];
+    if constexpr (kMixed) {
+      const float ih32 = rrcp<float>((float)hijd);
+      const double ih = (double)ih32 * fma(-hijd, (double)ih32, 2.0);
+      double r2 = 0, amax = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) { r2 = fma(ds[k], ds[k], r2); amax = fmax(amax, fabs(ds[k])); }
+      const double expt = r2 * (ih * ih);
+      const double nk = kInvSqrtPi * ih;
+      const double norm = (D == 3) ? nk * nk * nk : nk;
+      const double mj = s.m64[q];
+      const double ex = exp_neg_tab(expt, s.etab);
+      const double wm = norm * ex * mj;
+      if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
+      if (a.energy_smooth && !a.cut.outside(a.e_ratio * amax, hijd)) {
+        const double nke = kInvSqrtPi * (a.e_ratio * ih);
+        const double ne = (D == 3) ? nke * nke * nke : nke;
+        const double ee = a.e_sq16 ? pow16(ex) : exp_neg_tab(a.e_ratio * a.e_ratio * expt, s.etab);
+        const double vr = ne * ee * mj * s.irho64[q];
+        t.wn += vr;
+        t.en += vr * s.e64[q];
+      }
+      return;
+    }
     const R hij = (R)hijd;
     const R ih = rrcp<R>(hij);
     const R mj = s.m[q];
@@ -642,7 +819,8 @@ struct DensPass {
     }
     const R nk = R(kInvSqrtPi) * ih;
     const R norm = (D == 3) ? nk * nk * nk : nk;
-    const R wm = norm * rexp<R>(-expt) * mj;
+    const R ex = rexp<R>(-expt);
+    const R wm = norm * ex * mj;
     if (s.ok[q] == 1) t.r1 += (double)wm; else t.r2 += (double)wm;
     if (a.energy_smooth) {
       // weight(4 ds / h_ij, h_ij / 4) (HAVE_ENERGY_SMOOTH 1); exact FP64 box test
@@ -650,7 +828,7 @@ struct DensPass {
         const R nke = R(kInvSqrtPi) * (R(a.e_ratio) * ih);
         const R ne = (D == 3) ? nke * nke * nke : nke;
         const R e2 = R(a.e_ratio * a.e_ratio);
-        const R wm_e = ne * rexp<R>(-(e2 * expt)) * mj;
+        const R wm_e = ne * (a.e_sq16 ? pow16(ex) : rexp<R>(-(e2 * expt))) * mj;
         const R vr = wm_e / s.rho[q];
         t.wn += (double)vr;
         t.en += (double)(vr * s.e[q]);
@@ -693,11 +871,17 @@ struct VsmArgs {
 template <class R>
 struct VsmPass {
   using Args = VsmArgs;
+  template <class T> __device__ static void init(T &, int) {}
   struct Tile {
     R mr[kWave], vx[kWave], vy[kWave], vz[kWave];
     int ok[kWave];
   };
   struct Tgt { double hi, supp, v64[3], dv[3], wn; R ih, norm, v[3]; };
+  static constexpr int kSub = ::kSub;
+  __device__ static void combine(Tgt &t) {
+    for (int k = 0; k < 3; k++) t.dv[k] = sub_sum<kSub>(t.dv[k]);
+    t.wn = sub_sum<kSub>(t.wn);
+  }
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
@@ -764,11 +948,14 @@ struct AdaptArgs {
 template <class R>
 struct AdaptPass {
   using Args = AdaptArgs;
+  template <class T> __device__ static void init(T &, int) {}
   struct Tile {
     R m[kWave];
     int ok[kWave];
   };
   struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; bool done; };
+  static constexpr int kSub = ::kSub;
+  __device__ static void combine(Tgt &t) { t.rho_star = sub_sum<kSub>(t.rho_star); }
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &a, Tgt &t, int) {
     t.Hs = t.hcur * a.c_smooth;
@@ -819,6 +1006,7 @@ struct AdaptPass {
 // mode 0 counts, mode 1 writes device indices at offsets[i].
 // ---------------------------------------------------------------------------
 struct ListPass {
+  template <class T> __device__ static void init(T &, int) {}
   struct Args {
     int mode;
     uint32_t *count;
@@ -828,6 +1016,8 @@ struct ListPass {
   };
   struct Tile { int dummy; };
   struct Tgt { uint32_t n, o; double supp; };
+  static constexpr int kSub = 1;
+  __device__ static void combine(Tgt &) {}
   __device__ static int npasses(const Args &) { return 1; }
   __device__ static void begin_pass(const Args &, Tgt &, int) {}
   __device__ static void end_pass(const Args &, Tgt &, int, uint32_t, bool) {}
@@ -1346,19 +1536,20 @@ static int build_grid(psph_ctx *c) {
     c->cs_cap = need;
   }
   const size_t ub = c->n / kWave + (size_t)g.ncells + 1;
+  const size_t cub_ = c->n / kChunk + (size_t)g.ncells + 1;
   if (ub > c->tile_cap) {
     if (c->tbox) hipFree(c->tbox);
     c->tbox = nullptr;
     PSPH_TRY(dmalloc((void **)&c->tbox, ub * 2 * sizeof(float4)));
     c->tile_cap = ub;
   }
-  if (ub > c->chunk_cap) {
+  if (cub_ > c->chunk_cap) {
     if (c->cfirst) hipFree(c->cfirst);
     if (c->ccount) hipFree(c->ccount);
     c->cfirst = c->ccount = nullptr;
-    PSPH_TRY(dmalloc((void **)&c->cfirst, ub * 4));
-    PSPH_TRY(dmalloc((void **)&c->ccount, ub * 4));
-    c->chunk_cap = ub;
+    PSPH_TRY(dmalloc((void **)&c->cfirst, cub_ * 4));
+    PSPH_TRY(dmalloc((void **)&c->ccount, cub_ * 4));
+    c->chunk_cap = cub_;
   }
   return 0;
 }
@@ -1425,14 +1616,15 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   if (n) hipLaunchKernelGGL(k_refb, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.h,
                             c->cfg.ext_lo[0], c->cfg.ext_lo[1], c->cfg.ext_lo[2], del, nb[0], nb[1], nb[2],
                             c->refb, c->h_list);
-  // targets and their chunks (<= 64 consecutive targets of one cell)
+  // targets and their chunks (<= kChunk consecutive targets of one cell)
   hipLaunchKernelGGL(k_target_flags, blocks(n + 1), 256, 0, s, n, c->f.bc, c->flags);
   PSPH_TRY(excl_scan(c, c->flags, c->tpos, (size_t)n + 1));
   if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, c->flags, c->tpos, c->targets);
-  hipLaunchKernelGGL(k_cell_chunks, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tpos, c->cchunks);
+  hipLaunchKernelGGL(k_cell_chunks, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tpos, (uint32_t)kChunk,
+                     c->cchunks);
   PSPH_TRY(excl_scan(c, c->cchunks, c->coff, (size_t)g.ncells + 1));
   hipLaunchKernelGGL(k_chunks, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tpos, c->coff,
-                     c->cfirst, c->ccount, c->nchunks);
+                     (uint32_t)kChunk, c->cfirst, c->ccount, c->nchunks);
   HIP_TRY(hipGetLastError());
   tm.stop();
   c->grid_valid = true;
@@ -1464,16 +1656,17 @@ static PairCommon make_common(psph_ctx *c, double reach) {
   return pc;
 }
 
-// workgroups for a pair pass: an upper bound on the chunks (each cell may add
-// a partial chunk), padded so that kXcds * ceil(nchunks / kXcds) exist
-static unsigned chunk_grid(size_t n, size_t ncells) {
-  size_t ub = n / kWave + ncells + 1 + kXcds;
+// workgroups for a pair pass: an upper bound on the chunks of csz targets
+// (each cell may add a partial chunk), padded so that kXcds * ceil(nchunks /
+// kXcds) exist
+static unsigned chunk_grid(size_t n, size_t ncells, size_t csz) {
+  size_t ub = n / csz + ncells + 1 + kXcds;
   return (unsigned)((ub + kXcds - 1) / kXcds * kXcds);
 }
 
 template <class P>
 static void launch_pairs(psph_ctx *c, const PairCommon &pc, const typename P::Args &a) {
-  unsigned nb = chunk_grid(c->n, c->grid.ncells);
+  unsigned nb = chunk_grid(c->n, c->grid.ncells, kChunk);
   if (c->cfg.dim == 3)
     hipLaunchKernelGGL((k_pairs<3, Bound<3, P>>), nb, kWave, 0, c->stream, pc, a);
   else
@@ -1594,6 +1787,7 @@ extern "C" int psph_density(psph_ctx *c) {
   a.pghost = k.pghost_contrib; a.energy_smooth = k.energy_smooth; a.eos = eos_const(k);
   a.cut = cut_test(k.cutoff);
   a.o_wn = c->rec ? c->rec->d[RF_WNE] : nullptr;
+  a.e_sq16 = k.e_smooth_ratio * k.e_smooth_ratio == 16.0;
   {
     StageTimer tm(c, 4);
     if (n) launch_form<DensPass>(c, make_common(c, c->reach_main), a);
