@@ -177,7 +177,8 @@ struct AdjCommon {
   CutTest cut;
 };
 
-template <class P>
+// ROLES: 1 own role only, 2 neighbour role only, 3 both (one traversal).
+template <class P, int ROLES>
 __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], slf[kWave];
   __shared__ uint32_t srefb[kWave];
@@ -191,8 +192,10 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
   if (chunk >= nch) return;
   const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
   const int lane = threadIdx.x;
-  const bool active = (uint32_t)lane < cc;
-  const uint32_t i = pc.rows[cf + (active ? lane : 0)];
+  // kSub lanes per row, taking alternate candidates of each tile
+  const uint32_t slot = (uint32_t)lane / kSub, sub = (uint32_t)lane % kSub;
+  const bool active = slot < cc;
+  const uint32_t i = pc.rows[cf + (active ? slot : 0)];
 
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
   const uint32_t rb_i = pc.refb[i];
@@ -204,6 +207,8 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
   P::load_row(a, i, t);
   const ChunkBox box = chunk_box(pc.tr, xi);
   const GridDesc &g = pc.tr.g;
+  P::init(tile, lane);
+  __syncthreads();
 
   for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
     for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
@@ -226,15 +231,15 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
         const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
         if (active) {
           uint64_t mo = 0, mn = 0;
-          for (uint32_t q = 0; q < cnt; q++) {
+          for (uint32_t q = sub; q < cnt; q += kSub) {
             const double d0 = xi[0] - sx[q], d1 = xi[1] - sy[q], d2 = xi[2] - sz[q];
             const double amax = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
             const bool same = pc.list_all || srefb[q] == rb_i;
             const uint32_t jq = base + q;
-            if (tgt_i && P::own_ok(t, tile, q, jq, i) && (same || amax <= lf_i) && amax <= supp_i &&
-                !pc.cut.outside(amax, P::hker_own(hi, sh[q])))
+            if ((ROLES & 1) && tgt_i && P::own_ok(t, tile, q, jq, i) && (same || amax <= lf_i) &&
+                amax <= supp_i && !pc.cut.outside(amax, P::hker_own(hi, sh[q])))
               mo |= 1ull << q;
-            if (need_neigh(sbc[q]) && P::nbr_ok(t, tile, q, jq, i) && (same || amax <= slf[q]) &&
+            if ((ROLES & 2) && need_neigh(sbc[q]) && P::nbr_ok(t, tile, q, jq, i) && (same || amax <= slf[q]) &&
                 amax <= pc.cutoff2 * sh[q] && !pc.cut.outside(amax, P::hker_nbr(hi, sh[q])))
               mn |= 1ull << q;
           }
@@ -256,23 +261,37 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
         __syncthreads();
       }
     }
-  if (active) P::finish(a, t, i);
+  P::combine(t);
+  if (active && sub == 0) P::finish(a, t, i);
 }
 
 __device__ inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
+// 1 / x in FP64: hardware reciprocal refined by two Newton steps
+__device__ inline double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = r * fma(-x, r, 2.0);
+  return r * fma(-x, r, 2.0);
+}
+
 // --- velocity filter (smooth_velocity.cc:25-165) ----------------------------
 struct VsmAdj {
   struct Args {
-    const double *m, *rho, *v0, *v1, *v2;       // rho'' and v' of the step
+    const double *m, *rho, *h, *v0, *v1, *v2;   // rho'' and v' of the step
     const double *ax, *ay, *az, *ac;             // per-target seeds
     const int32_t *bc;
     LamPtrs l;
   };
-  struct Tile { double mr[kWave], v[3][kWave], a[3][kWave], c[kWave]; int ok[kWave]; };
-  struct Row { double mi, rhoi, mr, v[3], a[3], c, g[3], gv[3], gm, grho; bool ok; };
+  struct Tile {
+    double mr[kWave], ih2[kWave], nrm[kWave], v[3][kWave], a[3][kWave], c[kWave], etab[kETab];
+    int ok[kWave];
+  };
+  struct Row { double imi, irho, mr, ih2, nrm, v[3], a[3], c, g[3], gv[3], gm, grho; bool ok; };
+  __device__ static void init(Tile &s, int lane) { etab_fill(s.etab, lane); }
   __device__ static void load_row(const Args &A, uint32_t i, Row &t) {
-    t.mi = A.m[i]; t.rhoi = A.rho[i]; t.mr = t.mi / t.rhoi;
+    const double mi = A.m[i], rhoi = A.rho[i], ih = 1.0 / A.h[i], nk = kInvSqrtPi * ih;
+    t.imi = 1.0 / mi; t.irho = 1.0 / rhoi; t.mr = mi * t.irho;
+    t.ih2 = ih * ih; t.nrm = nk * nk * nk;
     t.v[0] = A.v0[i]; t.v[1] = A.v1[i]; t.v[2] = A.v2[i];
     t.a[0] = A.ax[i]; t.a[1] = A.ay[i]; t.a[2] = A.az[i]; t.c = A.ac[i];
     for (int k = 0; k < 3; k++) t.g[k] = t.gv[k] = 0;
@@ -280,7 +299,9 @@ struct VsmAdj {
     t.ok = contr_vel(A.bc[i]);
   }
   __device__ static void stage(const Args &A, uint32_t j, Tile &s, int l) {
+    const double ih = 1.0 / A.h[j], nk = kInvSqrtPi * ih;
     s.mr[l] = A.m[j] / A.rho[j];
+    s.ih2[l] = ih * ih; s.nrm[l] = nk * nk * nk;
     s.v[0][l] = A.v0[j]; s.v[1][l] = A.v1[j]; s.v[2][l] = A.v2[j];
     s.a[0][l] = A.ax[j]; s.a[1][l] = A.ay[j]; s.a[2][l] = A.az[j]; s.c[l] = A.ac[j];
     s.ok[l] = contr_vel(A.bc[j]);
@@ -290,26 +311,27 @@ struct VsmAdj {
   __device__ static bool own_ok(const Row &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   __device__ static bool nbr_ok(const Row &t, const Tile &, uint32_t, uint32_t, uint32_t) { return t.ok; }
   __device__ static void own(const Args &, Row &t, const Tile &s, uint32_t q, const double *dx, double,
-                             double hs, double) {
-    const double ih2 = 1.0 / (hs * hs);
-    const double nk = kInvSqrtPi / hs;
-    const double w = nk * nk * nk * exp(-dot3(dx, dx) * ih2) * s.mr[q];
+                             double, double) {
+    const double w = t.nrm * exp_neg_tab(dot3(dx, dx) * t.ih2, s.etab) * s.mr[q];
     const double gw = t.a[0] * s.v[0][q] + t.a[1] * s.v[1][q] + t.a[2] * s.v[2][q] - t.c;
-    const double f = -2.0 * gw * w * ih2;
+    const double f = -2.0 * gw * w * t.ih2;
     for (int k = 0; k < 3; k++) t.g[k] += f * dx[k];
   }
   __device__ static void nbr(const Args &, Row &t, const Tile &s, uint32_t q, const double *dx, double,
-                             double, double hs) {
-    const double ih2 = 1.0 / (hs * hs);
-    const double nk = kInvSqrtPi / hs;
-    const double w = nk * nk * nk * exp(-dot3(dx, dx) * ih2) * t.mr;
+                             double, double) {
+    const double ih2 = s.ih2[q];
+    const double w = s.nrm[q] * exp_neg_tab(dot3(dx, dx) * ih2, s.etab) * t.mr;
     const double a[3] = {s.a[0][q], s.a[1][q], s.a[2][q]};
     const double gw = dot3(a, t.v) - s.c[q];
     for (int k = 0; k < 3; k++) t.gv[k] += a[k] * w;
-    t.gm += gw * w / t.mi;
-    t.grho -= gw * w / t.rhoi;
+    t.gm += gw * w * t.imi;
+    t.grho -= gw * w * t.irho;
     const double f = -2.0 * gw * w * ih2;
     for (int k = 0; k < 3; k++) t.g[k] -= f * dx[k];
+  }
+  __device__ static void combine(Row &t) {
+    for (int k = 0; k < 3; k++) { t.g[k] = sub_sum<kSub>(t.g[k]); t.gv[k] = sub_sum<kSub>(t.gv[k]); }
+    t.gm = sub_sum<kSub>(t.gm); t.grho = sub_sum<kSub>(t.grho);
   }
   __device__ static void finish(const Args &A, Row &t, uint32_t i) {
     for (int k = 0; k < 3; k++) { A.l.f[L_X + k][i] += t.g[k]; A.l.f[L_VX + k][i] += t.gv[k]; }
@@ -325,14 +347,18 @@ struct DensAdj {
     const int32_t *bc, *phase;
     const double *s1, *s2, *sen, *swn;           // per-target seeds
     double e_ratio;
-    int pghost, energy_smooth;
+    int pghost, energy_smooth, e_sq16;
     CutTest cut;
     LamPtrs l;
   };
-  struct Tile { double m[kWave], rho[kWave], e1[kWave], s1[kWave], s2[kWave], sen[kWave], swn[kWave]; int ok[kWave]; };
-  struct Row { double mi, rhoi, e1i, s1, s2, sen, swn, g[3], gm, grho, ge; int ok; };
+  struct Tile {
+    double m[kWave], irho[kWave], e1[kWave], s1[kWave], s2[kWave], sen[kWave], swn[kWave], etab[kETab];
+    int ok[kWave];
+  };
+  struct Row { double mi, imi, irho, e1i, s1, s2, sen, swn, g[3], gm, grho, ge; int ok; };
+  __device__ static void init(Tile &s, int lane) { etab_fill(s.etab, lane); }
   __device__ static void load_row(const Args &A, uint32_t i, Row &t) {
-    t.mi = A.m[i]; t.rhoi = A.rho[i]; t.e1i = A.e1[i];
+    t.mi = A.m[i]; t.imi = 1.0 / t.mi; t.irho = 1.0 / A.rho[i]; t.e1i = A.e1[i];
     t.s1 = A.s1[i]; t.s2 = A.s2[i]; t.sen = A.sen[i]; t.swn = A.swn[i];
     const int ph = A.phase[i];
     t.ok = (contr_dens(A.pghost, A.bc[i]) && (ph == 1 || ph == 2)) ? ph : 0;
@@ -340,7 +366,7 @@ struct DensAdj {
     t.gm = t.grho = t.ge = 0;
   }
   __device__ static void stage(const Args &A, uint32_t j, Tile &s, int l) {
-    s.m[l] = A.m[j]; s.rho[l] = A.rho[j]; s.e1[l] = A.e1[j];
+    s.m[l] = A.m[j]; s.irho[l] = 1.0 / A.rho[j]; s.e1[l] = A.e1[j];
     s.s1[l] = A.s1[j]; s.s2[l] = A.s2[j]; s.sen[l] = A.sen[j]; s.swn[l] = A.swn[j];
     const int ph = A.phase[j];
     s.ok[l] = (contr_dens(A.pghost, A.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
@@ -349,46 +375,58 @@ struct DensAdj {
   __device__ static double hker_nbr(double hi, double hj) { return 0.5 * (hj + hi); }
   __device__ static bool own_ok(const Row &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   __device__ static bool nbr_ok(const Row &t, const Tile &, uint32_t, uint32_t, uint32_t) { return t.ok != 0; }
-  // dF/d(r^2) of the pair, and the energy weight terms
+  // kernel values of the pair: wm / m_j (density) and the energy weight / (m_j / rho_j)
+  __device__ static void weights(const Args &A, const Tile &s, double r2, double amax, double h, double &w,
+                                 double &we, double &ih2, bool &ebox) {
+    const double ih = rcp64(h);
+    ih2 = ih * ih;
+    const double nk = kInvSqrtPi * ih;
+    const double ex = exp_neg_tab(r2 * ih2, s.etab);
+    w = nk * nk * nk * ex;
+    ebox = A.energy_smooth && !A.cut.outside(A.e_ratio * amax, h);
+    we = 0;
+    if (ebox) {
+      const double nke = nk * A.e_ratio;
+      we = nke * nke * nke * (A.e_sq16 ? pow16(ex) : exp_neg_tab(A.e_ratio * A.e_ratio * r2 * ih2, s.etab));
+    }
+  }
   __device__ static void own(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
                              double hi, double hj) {
-    const double h = 0.5 * (hi + hj);
-    const double ih2 = 1.0 / (h * h);
-    const double r2 = dot3(dx, dx);
-    const double nk = kInvSqrtPi / h;
-    const double wm = nk * nk * nk * exp(-r2 * ih2) * s.m[q];
+    double w, we, ih2;
+    bool ebox;
+    weights(A, s, dot3(dx, dx), amax, 0.5 * (hi + hj), w, we, ih2, ebox);
+    const double wm = w * s.m[q];
     const double lr = s.ok[q] == 1 ? t.s1 : t.s2;
     double f = lr * wm * ih2;
-    if (A.energy_smooth && !A.cut.outside(A.e_ratio * amax, h)) {
-      const double e2 = A.e_ratio * A.e_ratio;
-      const double nke = kInvSqrtPi * A.e_ratio / h;
-      const double vr = nke * nke * nke * exp(-e2 * r2 * ih2) * s.m[q] / s.rho[q];
+    if (ebox) {
+      const double vr = we * s.m[q] * s.irho[q];
       const double gvr = t.swn + t.sen * s.e1[q];
-      f += gvr * vr * e2 * ih2;
+      f += gvr * vr * (A.e_ratio * A.e_ratio) * ih2;
     }
     for (int k = 0; k < 3; k++) t.g[k] -= 2.0 * f * dx[k];
   }
   __device__ static void nbr(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
                              double hi, double hj) {
-    const double h = 0.5 * (hj + hi);
-    const double ih2 = 1.0 / (h * h);
-    const double r2 = dot3(dx, dx);
-    const double nk = kInvSqrtPi / h;
-    const double wm = nk * nk * nk * exp(-r2 * ih2) * t.mi;
+    double w, we, ih2;
+    bool ebox;
+    weights(A, s, dot3(dx, dx), amax, 0.5 * (hj + hi), w, we, ih2, ebox);
+    const double wm = w * t.mi;
     const double lr = t.ok == 1 ? s.s1[q] : s.s2[q];
     double f = lr * wm * ih2;
-    t.gm += lr * wm / t.mi;
-    if (A.energy_smooth && !A.cut.outside(A.e_ratio * amax, h)) {
-      const double e2 = A.e_ratio * A.e_ratio;
-      const double nke = kInvSqrtPi * A.e_ratio / h;
-      const double vr = nke * nke * nke * exp(-e2 * r2 * ih2) * t.mi / t.rhoi;
+    t.gm += lr * w;
+    if (ebox) {
+      const double vr = we * t.mi * t.irho;
       const double gvr = s.swn[q] + s.sen[q] * t.e1i;
       t.ge += s.sen[q] * vr;
-      t.gm += gvr * vr / t.mi;
-      t.grho -= gvr * vr / t.rhoi;
-      f += gvr * vr * e2 * ih2;
+      t.gm += gvr * we * t.irho;
+      t.grho -= gvr * vr * t.irho;
+      f += gvr * vr * (A.e_ratio * A.e_ratio) * ih2;
     }
     for (int k = 0; k < 3; k++) t.g[k] += 2.0 * f * dx[k];
+  }
+  __device__ static void combine(Row &t) {
+    for (int k = 0; k < 3; k++) t.g[k] = sub_sum<kSub>(t.g[k]);
+    t.gm = sub_sum<kSub>(t.gm); t.grho = sub_sum<kSub>(t.grho); t.ge = sub_sum<kSub>(t.ge);
   }
   __device__ static void finish(const Args &A, Row &t, uint32_t i) {
     for (int k = 0; k < 3; k++) A.l.f[L_X + k][i] += t.g[k];
@@ -400,31 +438,36 @@ struct DensAdj {
 
 // --- momentum / energy RHS (mom_engr_update.cc:30-439) ----------------------
 struct RhsAdjConst { double alpha, beta, ata, eps_half, prandtl; int lans; };
-struct RhsSide { double m, rho, p, c, T, cp, vel[3]; };
+struct RhsSide { double m, im, rho, V, p, c, T, cp, vel[3]; };
 struct RhsGrad { double dx[3], vij[3], m_n, rho_s, rho_n, p_s, p_n, c_s, c_n, T_s, T_n, cp_s, cp_n; };
 
 // Reverse of one pair term of the summing particle s with neighbour n,
 // dx = x_s - x_n, seeds av = dt lambda_v'_s, ae = dt lambda_e'_s:
 //   L = av . (-M dW) + ae (M dW.v_sn / 2 + heat),  dW = G dx,
 //   G = -2 W / h^2, M = m_n (p_n / rho_n^2 + p_s / rho_s^2 + Pi).
-__device__ inline void rhs_pair_adj(const RhsAdjConst &k, const double *dx, double h, const RhsSide &S,
-                                    const RhsSide &N, const double *av, double ae, RhsGrad &g) {
-  const double ih2 = 1.0 / (h * h);
+// Quotients share reciprocals: 1/h, 1/(den rho_ab) for the viscosity and
+// 1/(sum|v_k dx_k| h c_ab) for the heat term.
+__device__ inline void rhs_pair_adj(const RhsAdjConst &k, const double *etab, const double *dx, double h,
+                                    const RhsSide &S, const RhsSide &N, const double *av, double ae, RhsGrad &g) {
+  const double ih = rcp64(h), ih2 = ih * ih;
   const double r2 = dot3(dx, dx);
-  const double nk = kInvSqrtPi / h;
-  const double W = nk * nk * nk * exp(-r2 * ih2);
+  const double nk = kInvSqrtPi * ih;
+  const double W = nk * nk * nk * exp_neg_tab(r2 * ih2, etab);
   const double G = -2.0 * ih2 * W;
   const double vij[3] = {S.vel[0] - N.vel[0], S.vel[1] - N.vel[1], S.vel[2] - N.vel[2]};
   const double vrab = dot3(dx, vij), dotvv = dot3(vij, vij);
-  const double Vn = 1.0 / N.rho, Vs = 1.0 / S.rho;
+  const double Vn = N.V, Vs = S.V;
   const double cab = 0.5 * (S.c + N.c);
   const bool avis = vrab < 0;
-  double vis_a = 0, mu = 0, rhoab = 1, den = 1;
+  double vis_a = 0, mu = 0, irhoab = 0, iden = 0;
   if (avis) {
-    rhoab = 0.5 * (S.rho + N.rho);
-    den = r2 + k.ata * h * h;
-    mu = h * vrab / den;
-    vis_a = (k.beta * mu - k.alpha * cab) * mu / rhoab;
+    const double rhoab = 0.5 * (S.rho + N.rho);
+    const double den = r2 + k.ata * h * h;
+    const double inv = rcp64(den * rhoab);
+    irhoab = den * inv;
+    iden = rhoab * inv;
+    mu = h * vrab * iden;
+    vis_a = (k.beta * mu - k.alpha * cab) * mu * irhoab;
   }
   double vis = vis_a;
   if (k.lans) vis -= k.eps_half * dotvv * Vn;
@@ -448,43 +491,47 @@ __device__ inline void rhs_pair_adj(const RhsAdjConst &k, const double *dx, doub
     g.rho_n += g_vis * k.eps_half * dotvv * Vn * Vn;
   }
   if (avis) {
-    const double g_mu = g_vis * (2.0 * k.beta * mu - k.alpha * cab) / rhoab;
-    g_cab -= g_vis * k.alpha * mu / rhoab;
-    const double g_rhoab = -g_vis * vis_a / rhoab;
+    const double g_mu = g_vis * (2.0 * k.beta * mu - k.alpha * cab) * irhoab;
+    g_cab -= g_vis * k.alpha * mu * irhoab;
+    const double g_rhoab = -g_vis * vis_a * irhoab;
     g.rho_s += 0.5 * g_rhoab;
     g.rho_n += 0.5 * g_rhoab;
-    g_vrab += g_mu * h / den;
-    g_r2 -= g_mu * mu / den;
+    g_vrab += g_mu * h * iden;
+    g_r2 -= g_mu * mu * iden;
   }
   g.T_s = g.T_n = g.cp_s = g.cp_n = 0;
   if (k.lans) {
-    // heat: m_n V_n V_s (T_s - T_n) 10 eps cp r^2 |v|^2 / (Pr sum|v_k dx_k| h c) (dW.dx / r^2)
+    // heat = base cp_ab |v|^2 Gd (T_s - T_n), Gd = G r^2 = dW.dx,
+    // base = 10 eps / Pr m_n V_n V_s / (sum|v_k dx_k| h c_ab)
     double dotvr = 0;
     for (int q = 0; q < 3; q++) dotvr += fabs(vij[q] * dx[q]);
     const double Gd = G * r2;
     if (dotvr != 0 && Gd != 0) {
       const double cpab = 0.5 * (S.cp + N.cp);
-      const double B = N.m * Vn * Vs * (10.0 * k.eps_half / k.prandtl) * cpab * dotvv * Gd / (dotvr * h * cab);
-      const double heat = B * (S.T - N.T);
+      const double inv = rcp64(dotvr * h * cab);
+      const double base = (10.0 * k.eps_half / k.prandtl) * N.m * Vn * Vs * inv;
+      const double B = base * cpab * dotvv * Gd;
+      const double dT = S.T - N.T;
+      const double hh = ae * B * dT;
+      const double aeb = ae * dT * base;
       g.T_s = ae * B;
       g.T_n = -ae * B;
-      const double hh = ae * heat;
-      g.m_n += hh / N.m;
+      g.m_n += hh * N.im;
       g.rho_n -= hh * Vn;
       g.rho_s -= hh * Vs;
-      g.cp_s = g.cp_n = 0.5 * hh / cpab;
-      g_dotvv += hh / dotvv;
-      const double g_Gd = hh / Gd;
+      g.cp_s = g.cp_n = 0.5 * aeb * dotvv * Gd;
+      g_dotvv += aeb * cpab * Gd;
+      const double g_Gd = aeb * cpab * dotvv;
       g_G += g_Gd * r2;
       g_r2 += g_Gd * G;
-      const double g_dotvr = -hh / dotvr;
+      const double g_dotvr = -hh * (inv * h * cab);
       for (int q = 0; q < 3; q++) {
         const double pr = vij[q] * dx[q];
         const double sg = pr > 0 ? 1.0 : (pr < 0 ? -1.0 : 0.0);
         g_vij[q] += g_dotvr * sg * dx[q];
         g_dx[q] += g_dotvr * sg * vij[q];
       }
-      g_cab -= hh / cab;
+      g_cab -= hh * (inv * dotvr * h);
     }
   }
   g_r2 -= g_G * G * ih2;
@@ -502,21 +549,28 @@ struct RhsAdj {
     RhsAdjConst k;
     LamPtrs l;
   };
-  struct Tile { double m[kWave], rho[kWave], p[kWave], c[kWave], T[kWave], cp[kWave], u[3][kWave], a[3][kWave], ae[kWave]; };
+  struct Tile {
+    double m[kWave], im[kWave], rho[kWave], V[kWave], p[kWave], c[kWave], T[kWave], cp[kWave], u[3][kWave];
+    double a[3][kWave], ae[kWave], etab[kETab];
+  };
   struct Row { RhsSide me; double a[3], ae, g[3], gu[3], gm, grho, gp, gc, gT, gcp; };
+  __device__ static void init(Tile &s, int lane) { etab_fill(s.etab, lane); }
   __device__ static void load_row(const Args &A, uint32_t i, Row &t) {
-    t.me = RhsSide{A.m[i], A.rho[i], A.p[i], A.c[i], A.T[i], A.cp[i], {A.u0[i], A.u1[i], A.u2[i]}};
+    const double m = A.m[i], rho = A.rho[i];
+    t.me = RhsSide{m, 1.0 / m, rho, 1.0 / rho, A.p[i], A.c[i], A.T[i], A.cp[i], {A.u0[i], A.u1[i], A.u2[i]}};
     t.a[0] = A.a0[i]; t.a[1] = A.a1[i]; t.a[2] = A.a2[i]; t.ae = A.ae[i];
     for (int k = 0; k < 3; k++) t.g[k] = t.gu[k] = 0;
     t.gm = t.grho = t.gp = t.gc = t.gT = t.gcp = 0;
   }
   __device__ static void stage(const Args &A, uint32_t j, Tile &s, int l) {
-    s.m[l] = A.m[j]; s.rho[l] = A.rho[j]; s.p[l] = A.p[j]; s.c[l] = A.c[j]; s.T[l] = A.T[j]; s.cp[l] = A.cp[j];
+    const double m = A.m[j], rho = A.rho[j];
+    s.m[l] = m; s.im[l] = 1.0 / m; s.rho[l] = rho; s.V[l] = 1.0 / rho;
+    s.p[l] = A.p[j]; s.c[l] = A.c[j]; s.T[l] = A.T[j]; s.cp[l] = A.cp[j];
     s.u[0][l] = A.u0[j]; s.u[1][l] = A.u1[j]; s.u[2][l] = A.u2[j];
     s.a[0][l] = A.a0[j]; s.a[1][l] = A.a1[j]; s.a[2][l] = A.a2[j]; s.ae[l] = A.ae[j];
   }
   __device__ static RhsSide side(const Tile &s, uint32_t q) {
-    return RhsSide{s.m[q], s.rho[q], s.p[q], s.c[q], s.T[q], s.cp[q], {s.u[0][q], s.u[1][q], s.u[2][q]}};
+    return RhsSide{s.m[q], s.im[q], s.rho[q], s.V[q], s.p[q], s.c[q], s.T[q], s.cp[q], {s.u[0][q], s.u[1][q], s.u[2][q]}};
   }
   __device__ static double hker_own(double hi, double hj) { return 0.5 * (hi + hj); }
   __device__ static double hker_nbr(double hi, double hj) { return 0.5 * (hj + hi); }
@@ -525,7 +579,7 @@ struct RhsAdj {
   __device__ static void own(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double,
                              double hi, double hj) {
     RhsGrad g;
-    rhs_pair_adj(A.k, dx, 0.5 * (hi + hj), t.me, side(s, q), t.a, t.ae, g);
+    rhs_pair_adj(A.k, s.etab, dx, 0.5 * (hi + hj), t.me, side(s, q), t.a, t.ae, g);
     for (int k = 0; k < 3; k++) { t.g[k] += g.dx[k]; t.gu[k] += g.vij[k]; }
     t.grho += g.rho_s; t.gp += g.p_s; t.gc += g.c_s; t.gT += g.T_s; t.gcp += g.cp_s;
   }
@@ -533,9 +587,14 @@ struct RhsAdj {
                              double hi, double hj) {
     RhsGrad g;
     const double a[3] = {s.a[0][q], s.a[1][q], s.a[2][q]};
-    rhs_pair_adj(A.k, dx, 0.5 * (hj + hi), side(s, q), t.me, a, s.ae[q], g);
+    rhs_pair_adj(A.k, s.etab, dx, 0.5 * (hj + hi), side(s, q), t.me, a, s.ae[q], g);
     for (int k = 0; k < 3; k++) { t.g[k] -= g.dx[k]; t.gu[k] -= g.vij[k]; }
     t.gm += g.m_n; t.grho += g.rho_n; t.gp += g.p_n; t.gc += g.c_n; t.gT += g.T_n; t.gcp += g.cp_n;
+  }
+  __device__ static void combine(Row &t) {
+    for (int k = 0; k < 3; k++) { t.g[k] = sub_sum<kSub>(t.g[k]); t.gu[k] = sub_sum<kSub>(t.gu[k]); }
+    t.gm = sub_sum<kSub>(t.gm); t.grho = sub_sum<kSub>(t.grho); t.gp = sub_sum<kSub>(t.gp);
+    t.gc = sub_sum<kSub>(t.gc); t.gT = sub_sum<kSub>(t.gT); t.gcp = sub_sum<kSub>(t.gcp);
   }
   __device__ static void finish(const Args &A, Row &t, uint32_t i) {
     const int u0 = A.k.lans ? L_SVX : L_VX;
@@ -545,33 +604,14 @@ struct RhsAdj {
   }
 };
 
-// cells holding at least one target (the benign race writes the same value)
-__global__ void k_cell_has_target(uint32_t n, const int32_t *bc, const uint32_t *ckey, int sh, uint32_t *tcell) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && need_neigh(bc[i])) tcell[ckey[i] >> sh] = 1u;
-}
-// Adjoint rows: real particles, erupt ghosts, and boundary ghosts (bc 1, 2:
-// constant in time, but their velocity and state enter their neighbours'
-// sums) whose cell neighbourhood holds a target.
-// rr: cells within the pass reach
-__global__ void k_adj_row_flags(uint32_t n, const int32_t *bc, const uint32_t *ckey, int sh, const uint32_t *tcell,
-                                int nx, int ny, int nz, int rr, uint32_t *flags) {
+// Adjoint rows: real particles, erupt ghosts, and the boundary ghosts (bc 1,
+// 2: constant in time, but their velocity and state enter their neighbours'
+// sums) that some target's sum met in the recorded step (gflag).
+__global__ void k_adj_row_flags(uint32_t n, const int32_t *bc, const uint32_t *gflag, uint32_t *flags) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   const int b = i < n ? bc[i] : -1;
-  uint32_t f = (is_real(b) || b == 0) ? 1u : 0u;
-  if (b == 1 || b == 2) {
-    const uint32_t k = ckey[i] >> sh;
-    const int cx = (int)(k % (uint32_t)nx), cy = (int)((k / (uint32_t)nx) % (uint32_t)ny), cz = (int)(k / ((uint32_t)nx * ny));
-    for (int dz = -rr; dz <= rr && !f; dz++)
-      for (int dy = -rr; dy <= rr && !f; dy++)
-        for (int dx = -rr; dx <= rr && !f; dx++) {
-          const int x = cx + dx, y = cy + dy, z = cz + dz;
-          if (x < 0 || y < 0 || z < 0 || x >= nx || y >= ny || z >= nz) continue;
-          if (tcell[((size_t)z * ny + y) * nx + x]) f = 1u;
-        }
-  }
-  flags[i] = f;
+  flags[i] = (is_real(b) || b == 0 || ((b == 1 || b == 2) && gflag[i])) ? 1u : 0u;
 }
 
 // fixed-order block sums of one array
@@ -652,9 +692,9 @@ static void snap_free(Snapshot &s) {
 
 static void rec_free(StepRec &r) {
   for (auto &p : r.d) { if (p) hipFree(p); p = nullptr; }
-  void *ptrs[] = {r.pid, r.refb, r.ckey, r.bc, r.phase, r.cs, r.toff, r.tbox};
+  void *ptrs[] = {r.pid, r.refb, r.ckey, r.gflag, r.bc, r.phase, r.cs, r.toff, r.tbox};
   for (auto p : ptrs) if (p) hipFree(p);
-  r.pid = r.refb = r.ckey = nullptr;
+  r.pid = r.refb = r.ckey = r.gflag = nullptr;
   r.bc = r.phase = nullptr;
   r.cs = r.toff = nullptr;
   r.tbox = nullptr;
@@ -739,7 +779,7 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
     if (n > r.cap) {
       const size_t cap = std::max(c->cap, n);
       for (auto &p : r.d) { if (p) hipFree(p); p = nullptr; }
-      uint32_t **u32s[] = {&r.pid, &r.refb, &r.ckey};
+      uint32_t **u32s[] = {&r.pid, &r.refb, &r.ckey, &r.gflag};
       for (auto pp : u32s) { if (*pp) hipFree(*pp); *pp = nullptr; }
       if (r.bc) hipFree(r.bc);
       if (r.phase) hipFree(r.phase);
@@ -780,6 +820,7 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
       HIP_TRY(hipMemcpyAsync(r.ckey, c->key_a, n * 4, hipMemcpyDeviceToDevice, st));
       HIP_TRY(hipMemcpyAsync(r.bc, c->f.bc, n * 4, hipMemcpyDeviceToDevice, st));
       HIP_TRY(hipMemcpyAsync(r.phase, c->f.phase, n * 4, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_fill_i32, blocks(n), 256, 0, st, (int *)r.gflag, 0, (int)n);
     }
     HIP_TRY(hipMemcpyAsync(r.cs, c->cs, ncs * 4, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipMemcpyAsync(r.toff, c->toff, ncs * 4, hipMemcpyDeviceToDevice, st));
@@ -858,7 +899,7 @@ static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
     PSPH_TRY(dmalloc((void **)&A.tcell, ((size_t)ncells + 1) * 4));
     A.tcell_cap = (size_t)ncells + 1;
   }
-  const size_t chunks_ub = n / kWave + (size_t)ncells + 1;
+  const size_t chunks_ub = n / kChunk + (size_t)ncells + 1;
   if (chunks_ub > A.chunk_cap) {
     void *ptrs[] = {A.cfirst, A.ccount};
     for (auto p : ptrs) if (p) hipFree(p);
@@ -867,25 +908,20 @@ static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
     A.chunk_cap = chunks_ub;
   }
   if (!A.nchunks) PSPH_TRY(dmalloc((void **)&A.nchunks, 4));
-  const int sh = 3 * g.fbits;
-  const int rr = g.inv_e > 0 ? (int)std::ceil(R.reach * g.inv_e) : 0;
-  hipLaunchKernelGGL(k_fill_i32, blocks(ncells), 256, 0, s, (int *)A.tcell, 0, (int)ncells);
-  if (n) hipLaunchKernelGGL(k_cell_has_target, blocks(n), 256, 0, s, n, R.bc, R.ckey, sh, A.tcell);
-  hipLaunchKernelGGL(k_adj_row_flags, blocks(n + 1), 256, 0, s, n, R.bc, R.ckey, sh, A.tcell, g.nx, g.ny, g.nz,
-                     rr, A.flags);
+  hipLaunchKernelGGL(k_adj_row_flags, blocks(n + 1), 256, 0, s, n, R.bc, R.gflag, A.flags);
   PSPH_TRY(excl_scan(c, A.flags, A.tpos, (size_t)n + 1));
   if (n) hipLaunchKernelGGL(k_targets, blocks(n), 256, 0, s, n, A.flags, A.tpos, A.rows);
-  hipLaunchKernelGGL(k_cell_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, (uint32_t)kWave, A.cchunks);
+  hipLaunchKernelGGL(k_cell_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, (uint32_t)kChunk, A.cchunks);
   PSPH_TRY(excl_scan(c, A.cchunks, A.coff, (size_t)ncells + 1));
   hipLaunchKernelGGL(k_chunks, blocks(ncells + 1), 256, 0, s, ncells, R.cs, A.tpos, A.coff,
-                     (uint32_t)kWave, A.cfirst, A.ccount, A.nchunks);
+                     (uint32_t)kChunk, A.cfirst, A.ccount, A.nchunks);
   HIP_TRY(hipGetLastError());
   return 0;
 }
 
-template <class P>
+template <class P, int ROLES = 3>
 static void launch_adj(psph_ctx *c, const AdjCommon &pc, const StepRec &R, const typename P::Args &a) {
-  hipLaunchKernelGGL(k_adj_pairs<P>, chunk_grid(R.n, R.grid.ncells, kWave), kWave, 0, c->stream, pc, a);
+  hipLaunchKernelGGL((k_adj_pairs<P, ROLES>), chunk_grid(R.n, R.grid.ncells, kChunk), kWave, 0, c->stream, pc, a);
 }
 
 struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };
@@ -956,7 +992,8 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
   if (k.lans) {
     hipLaunchKernelGGL(k_adj_vsm_pre, blocks(n), 256, 0, st, n, R.bc, R.d[RF_WNV], R.d[RF_SV2X], R.d[RF_SV2Y],
                        R.d[RF_SV2Z], l, A.aux[0], A.aux[1], A.aux[2], A.aux[3]);
-    VsmAdj::Args a{R.d[RF_M], R.d[RF_RHO2], R.d[RF_V1X], R.d[RF_V1Y], R.d[RF_V1Z], A.aux[0], A.aux[1], A.aux[2], A.aux[3], R.bc, l};
+    VsmAdj::Args a{R.d[RF_M], R.d[RF_RHO2], R.d[RF_H], R.d[RF_V1X], R.d[RF_V1Y], R.d[RF_V1Z], A.aux[0], A.aux[1],
+                   A.aux[2], A.aux[3], R.bc, l};
     launch_adj<VsmAdj>(c, pc, R, a);
   }
   // 4. density + EOS
@@ -967,7 +1004,8 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
     PSPH_TRY(dev_sum(c, A, A.aux[4], n, &gn));
     th.ng0 += gn;
     DensAdj::Args a{R.d[RF_M], R.d[RF_RHO], R.d[RF_E1], R.bc, R.phase, A.aux[0], A.aux[1], A.aux[2], A.aux[3],
-                    k.e_smooth_ratio, k.pghost_contrib, k.energy_smooth, cut_test(k.cutoff), l};
+                    k.e_smooth_ratio, k.pghost_contrib, k.energy_smooth,
+                    k.e_smooth_ratio * k.e_smooth_ratio == 16.0 ? 1 : 0, cut_test(k.cutoff), l};
     launch_adj<DensAdj>(c, pc, R, a);
   }
   // 5. momentum / energy

@@ -321,6 +321,8 @@ struct PairCommon {
   const double *x, *y, *z;
   const uint32_t *refb;
   const double *h_list;
+  const int32_t *bc;
+  uint32_t *gflag;            // optional: marks boundary ghosts (bc 1, 2) met as neighbours
   Traversal tr;
   const uint32_t *targets;
   const uint32_t *cfirst, *ccount, *nchunks;
@@ -350,6 +352,7 @@ template <int D, class P>
 __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
   __shared__ uint32_t srefb[kWave];
+  __shared__ int sgh[kWave];
   __shared__ typename P::Tile tile;
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
   // XCD x runs workgroups x, x+8, ...; give it the contiguous chunk range
@@ -390,6 +393,7 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
             if (j < hi) {
               sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
               srefb[lane] = pc.refb[j];
+              if (pc.gflag) { const int b = pc.bc[j]; sgh[lane] = b == 1 || b == 2; }
               P::stage(a, j, tile, lane);
             } else {
               // empty slot: a position no predicate accepts (|dx| > every support)
@@ -428,6 +432,7 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
                 dx[0] = xi[0] - sx[q];
                 if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
                 P::pair(a, t, tile, q, base + q, i, dx);
+                if (pc.gflag && sgh[q]) pc.gflag[base + q] = 1u;
               }
             }
             __syncthreads();
@@ -1196,6 +1201,7 @@ struct StepRec {
   size_t n = 0, cap = 0;
   double *d[RF_N] = {};
   uint32_t *pid = nullptr, *refb = nullptr, *ckey = nullptr;
+  uint32_t *gflag = nullptr;                  // boundary ghosts met as neighbours in this step
   int32_t *bc = nullptr, *phase = nullptr;
   uint32_t *cs = nullptr, *toff = nullptr;   // cell starts and first tile of each cell
   float4 *tbox = nullptr;                     // tile boxes
@@ -1646,6 +1652,8 @@ static PairCommon make_common(psph_ctx *c, double reach) {
   PairCommon pc;
   pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z;
   pc.refb = c->refb; pc.h_list = c->h_list;
+  pc.bc = c->f.bc;
+  pc.gflag = c->rec ? c->rec->gflag : nullptr;
   pc.tr = traversal(c, reach);
   pc.targets = c->targets;
   pc.cfirst = c->cfirst; pc.ccount = c->ccount; pc.nchunks = c->nchunks;

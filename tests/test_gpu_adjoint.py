@@ -172,7 +172,9 @@ def test_smoothed_velocity_gradient_in_motion():
         real = np.nonzero(Q["bc"] == psph.BC_REAL)[0]
         rng = np.random.default_rng(SEED + 2000)
         pick = list(real[np.argsort(-np.abs(g[real]).max(1))[:4]]) + list(rng.choice(real, 4, replace=False))
-        eps = 1e-6 * cfg.vent_v
+        # particles with dH/dsv ~ 1e-4 of the largest move H by ~1e-11 m at this
+        # step: 1e-5 relative keeps the difference well above FP64 round-off of H
+        eps = 1e-5 * cfg.vent_v
         worst = 0.0
         for i in pick:
             for k, f in enumerate(("svx", "svy", "svz")):
@@ -184,7 +186,7 @@ def test_smoothed_velocity_gradient_in_motion():
                     S.replay(6)
                     hs.append(S.diag_height())
                 fdv = (hs[0] - hs[1]) / (2 * eps)
-                err = abs(g[i, k] - fdv) / max(abs(fdv), 1e-3 * np.abs(g).max())
+                err = abs(g[i, k] - fdv) / max(abs(fdv), 1e-2 * np.abs(g).max())
                 worst = max(worst, err)
                 print(f"particle {i} {f}: AD {g[i, k]:.9e} FD {fdv:.9e} rel {err:.2e}", flush=True)
         assert worst <= FD_TOL, worst
