@@ -194,10 +194,12 @@ def adjoint_leg(args):
     log(f"[bench] adjoint: {len(P)} particles, {N} steps, ckpt {K}: forward {fwd_ms:.0f} ms, "
         f"recorded forward {r['forward_ms']:.0f} ms, backward {r['backward_ms']:.0f} ms")
     fd_full = fd_checks(N, r, (1e-6, 1e-10))
-    n_short = min(args.adjoint_short, N)
-    S.load(P)
-    r_short = S.adjoint_run(n_short, min(K, n_short), grads=False)
-    fd_short = fd_checks(n_short, r_short, (1e-6, 1e-8))
+    fd_short = []
+    if args.adjoint_fd > 0:
+        n_short = min(args.adjoint_short, N)
+        S.load(P)
+        r_short = S.adjoint_run(n_short, min(K, n_short), grads=False)
+        fd_short = fd_checks(n_short, r_short, (1e-6, 1e-8))
     S.close()
     short_err = max((x["rel_err_eps_1e-08"] or 0) for x in fd_short) if fd_short else None
     return {"workload": f"weak plume, {args.adjoint_particles / 1e3:g}k real particles, {N} steps, "
@@ -254,6 +256,7 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=3, help="vent parameters checked by central differences")
     ap.add_argument("--adjoint-short", type=int, default=20, help="horizon of the short FD check")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB: run only a plain device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -272,6 +275,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
+
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(args)}))
         return
 
     base = weak_plume_config(args.particles)

@@ -164,8 +164,9 @@ __global__ void k_adj_rhs_pre(uint32_t n, const int32_t *bc, double dt, double g
 // inputs (real particles, erupt ghosts, boundary ghosts next to targets).  For row i and
 // candidate j it evaluates the forward pass's exact pair predicate twice:
 // with i as the summing particle (own role, i a target) and with j as the
-// summing particle (neighbour role, j a target).  dx passed to own() is
-// x_i - x_j, to nbr() x_j - x_i: always summing minus neighbour.
+// summing particle (neighbour role, j a target), and calls pair() once with
+// both flags and dx = x_i - x_j.  Within a pass, dx given to an own()/nbr()
+// role routine is always summing minus neighbour.
 // ---------------------------------------------------------------------------
 struct AdjCommon {
   const double *x, *y, *z, *h, *h_list;
@@ -243,19 +244,15 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
                 amax <= pc.cutoff2 * sh[q] && !pc.cut.outside(amax, P::hker_nbr(hi, sh[q])))
               mn |= 1ull << q;
           }
-          while (mo) {
-            const uint32_t q = (uint32_t)__builtin_ctzll(mo);
-            mo &= mo - 1;
+          // one evaluation per candidate met in either role: the pair's
+          // kernel and viscosity geometry is shared by both roles
+          uint64_t mb = mo | mn;
+          while (mb) {
+            const uint32_t q = (uint32_t)__builtin_ctzll(mb);
+            mb &= mb - 1;
             const double dx[3] = {xi[0] - sx[q], xi[1] - sy[q], xi[2] - sz[q]};
             const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-            P::own(a, t, tile, q, dx, amax, hi, sh[q]);
-          }
-          while (mn) {
-            const uint32_t q = (uint32_t)__builtin_ctzll(mn);
-            mn &= mn - 1;
-            const double dx[3] = {sx[q] - xi[0], sy[q] - xi[1], sz[q] - xi[2]};
-            const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-            P::nbr(a, t, tile, q, dx, amax, hi, sh[q]);
+            P::pair(a, t, tile, q, dx, amax, hi, sh[q], (mo >> q) & 1, (mn >> q) & 1);
           }
         }
         __syncthreads();
@@ -329,6 +326,15 @@ struct VsmAdj {
     const double f = -2.0 * gw * w * ih2;
     for (int k = 0; k < 3; k++) t.g[k] -= f * dx[k];
   }
+  // the two roles use different kernels (h_i and h_j): nothing to share
+  __device__ static void pair(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
+                              double hi, double hj, bool fo, bool fn) {
+    if (fo) own(A, t, s, q, dx, amax, hi, hj);
+    if (fn) {
+      const double dn[3] = {-dx[0], -dx[1], -dx[2]};
+      nbr(A, t, s, q, dn, amax, hi, hj);
+    }
+  }
   __device__ static void combine(Row &t) {
     for (int k = 0; k < 3; k++) { t.g[k] = sub_sum<kSub>(t.g[k]); t.gv[k] = sub_sum<kSub>(t.gv[k]); }
     t.gm = sub_sum<kSub>(t.gm); t.grho = sub_sum<kSub>(t.grho);
@@ -390,39 +396,39 @@ struct DensAdj {
       we = nke * nke * nke * (A.e_sq16 ? pow16(ex) : exp_neg_tab(A.e_ratio * A.e_ratio * r2 * ih2, s.etab));
     }
   }
-  __device__ static void own(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
-                             double hi, double hj) {
+  // h_ij is symmetric: both roles share the pair's two kernel weights.
+  // dx = x_i - x_j; the neighbour role's position term, 2 f (x_j - x_i) added,
+  // is the same subtraction as the own role's.
+  __device__ static void pair(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
+                              double hi, double hj, bool fo, bool fn) {
     double w, we, ih2;
     bool ebox;
     weights(A, s, dot3(dx, dx), amax, 0.5 * (hi + hj), w, we, ih2, ebox);
-    const double wm = w * s.m[q];
-    const double lr = s.ok[q] == 1 ? t.s1 : t.s2;
-    double f = lr * wm * ih2;
-    if (ebox) {
-      const double vr = we * s.m[q] * s.irho[q];
-      const double gvr = t.swn + t.sen * s.e1[q];
-      f += gvr * vr * (A.e_ratio * A.e_ratio) * ih2;
+    double f = 0;
+    if (fo) {
+      const double lr = s.ok[q] == 1 ? t.s1 : t.s2;
+      f = lr * (w * s.m[q]) * ih2;
+      if (ebox) {
+        const double vr = we * s.m[q] * s.irho[q];
+        const double gvr = t.swn + t.sen * s.e1[q];
+        f += gvr * vr * (A.e_ratio * A.e_ratio) * ih2;
+      }
+      for (int k = 0; k < 3; k++) t.g[k] -= 2.0 * f * dx[k];
     }
-    for (int k = 0; k < 3; k++) t.g[k] -= 2.0 * f * dx[k];
-  }
-  __device__ static void nbr(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double amax,
-                             double hi, double hj) {
-    double w, we, ih2;
-    bool ebox;
-    weights(A, s, dot3(dx, dx), amax, 0.5 * (hj + hi), w, we, ih2, ebox);
-    const double wm = w * t.mi;
-    const double lr = t.ok == 1 ? s.s1[q] : s.s2[q];
-    double f = lr * wm * ih2;
-    t.gm += lr * w;
-    if (ebox) {
-      const double vr = we * t.mi * t.irho;
-      const double gvr = s.swn[q] + s.sen[q] * t.e1i;
-      t.ge += s.sen[q] * vr;
-      t.gm += gvr * we * t.irho;
-      t.grho -= gvr * vr * t.irho;
-      f += gvr * vr * (A.e_ratio * A.e_ratio) * ih2;
+    if (fn) {
+      const double lr = t.ok == 1 ? s.s1[q] : s.s2[q];
+      f = lr * (w * t.mi) * ih2;
+      t.gm += lr * w;
+      if (ebox) {
+        const double vr = we * t.mi * t.irho;
+        const double gvr = s.swn[q] + s.sen[q] * t.e1i;
+        t.ge += s.sen[q] * vr;
+        t.gm += gvr * we * t.irho;
+        t.grho -= gvr * vr * t.irho;
+        f += gvr * vr * (A.e_ratio * A.e_ratio) * ih2;
+      }
+      for (int k = 0; k < 3; k++) t.g[k] -= 2.0 * f * dx[k];
     }
-    for (int k = 0; k < 3; k++) t.g[k] += 2.0 * f * dx[k];
   }
   __device__ static void combine(Row &t) {
     for (int k = 0; k < 3; k++) t.g[k] = sub_sum<kSub>(t.g[k]);
@@ -445,41 +451,66 @@ struct RhsGrad { double dx[3], vij[3], m_n, rho_s, rho_n, p_s, p_n, c_s, c_n, T_
 // dx = x_s - x_n, seeds av = dt lambda_v'_s, ae = dt lambda_e'_s:
 //   L = av . (-M dW) + ae (M dW.v_sn / 2 + heat),  dW = G dx,
 //   G = -2 W / h^2, M = m_n (p_n / rho_n^2 + p_s / rho_s^2 + Pi).
-// Quotients share reciprocals: 1/h, 1/(den rho_ab) for the viscosity and
-// 1/(sum|v_k dx_k| h c_ab) for the heat term.
-__device__ inline void rhs_pair_adj(const RhsAdjConst &k, const double *etab, const double *dx, double h,
-                                    const RhsSide &S, const RhsSide &N, const double *av, double ae, RhsGrad &g) {
-  const double ih = rcp64(h), ih2 = ih * ih;
-  const double r2 = dot3(dx, dx);
+// The pair's geometry (kernel, viscosity switch and mu, heat denominator) is
+// symmetric in s <-> n (dx and v_sn both change sign), so it is computed once
+// (RhsGeo) and serves both roles.  Quotients share reciprocals: 1/h,
+// 1/(den rho_ab) for the viscosity and 1/(sum|v_k dx_k| h c_ab) for the heat.
+struct RhsGeo {
+  double h, ih2, r2, G, vrab, dotvv, cab, mu, irhoab, iden, vis_a, dotvr, Gd, cpab, hinv;
+  bool avis, heat;
+};
+
+__device__ inline RhsGeo rhs_geo(const RhsAdjConst &k, const double *etab, const double *dx, const double *vij,
+                                 double h, const RhsSide &S, const RhsSide &N) {
+  RhsGeo q;
+  q.h = h;
+  const double ih = rcp64(h);
+  q.ih2 = ih * ih;
+  q.r2 = dot3(dx, dx);
   const double nk = kInvSqrtPi * ih;
-  const double W = nk * nk * nk * exp_neg_tab(r2 * ih2, etab);
-  const double G = -2.0 * ih2 * W;
-  const double vij[3] = {S.vel[0] - N.vel[0], S.vel[1] - N.vel[1], S.vel[2] - N.vel[2]};
-  const double vrab = dot3(dx, vij), dotvv = dot3(vij, vij);
-  const double Vn = N.V, Vs = S.V;
-  const double cab = 0.5 * (S.c + N.c);
-  const bool avis = vrab < 0;
-  double vis_a = 0, mu = 0, irhoab = 0, iden = 0;
-  if (avis) {
+  const double W = nk * nk * nk * exp_neg_tab(q.r2 * q.ih2, etab);
+  q.G = -2.0 * q.ih2 * W;
+  q.vrab = dot3(dx, vij);
+  q.dotvv = dot3(vij, vij);
+  q.cab = 0.5 * (S.c + N.c);
+  q.avis = q.vrab < 0;
+  q.vis_a = q.mu = q.irhoab = q.iden = 0;
+  if (q.avis) {
     const double rhoab = 0.5 * (S.rho + N.rho);
-    const double den = r2 + k.ata * h * h;
+    const double den = q.r2 + k.ata * h * h;
     const double inv = rcp64(den * rhoab);
-    irhoab = den * inv;
-    iden = rhoab * inv;
-    mu = h * vrab * iden;
-    vis_a = (k.beta * mu - k.alpha * cab) * mu * irhoab;
+    q.irhoab = den * inv;
+    q.iden = rhoab * inv;
+    q.mu = h * q.vrab * q.iden;
+    q.vis_a = (k.beta * q.mu - k.alpha * q.cab) * q.mu * q.irhoab;
   }
-  double vis = vis_a;
-  if (k.lans) vis -= k.eps_half * dotvv * Vn;
+  q.dotvr = 0;
+  for (int d = 0; d < 3; d++) q.dotvr += fabs(vij[d] * dx[d]);
+  q.Gd = q.G * q.r2;
+  q.heat = k.lans && q.dotvr != 0 && q.Gd != 0;
+  q.cpab = q.hinv = 0;
+  if (q.heat) {
+    q.cpab = 0.5 * (S.cp + N.cp);
+    q.hinv = rcp64(q.dotvr * h * q.cab);
+  }
+  return q;
+}
+
+// The role-dependent part, with dx and vij oriented summing minus neighbour.
+__device__ inline void rhs_role(const RhsAdjConst &k, const RhsGeo &q, const double *dx, const double *vij,
+                                const RhsSide &S, const RhsSide &N, const double *av, double ae, RhsGrad &g) {
+  const double Vn = N.V, Vs = S.V;
+  double vis = q.vis_a;
+  if (k.lans) vis -= k.eps_half * q.dotvv * Vn;
   const double PV2n = N.p * Vn * Vn, PV2s = S.p * Vs * Vs;
   const double Msum = PV2n + PV2s + vis;
   const double mpv = N.m * Msum;
-  const double Q = -dot3(av, dx) + 0.5 * ae * vrab;
-  const double g_mpv = G * Q, g_Q = mpv * G;
+  const double Q = -dot3(av, dx) + 0.5 * ae * q.vrab;
+  const double g_mpv = q.G * Q, g_Q = mpv * q.G;
   double g_G = mpv * Q;
   double g_r2 = 0, g_dotvv = 0, g_cab = 0, g_vrab = 0.5 * ae * g_Q;
   double g_dx[3], g_vij[3] = {0, 0, 0};
-  for (int q = 0; q < 3; q++) g_dx[q] = -g_Q * av[q];
+  for (int d = 0; d < 3; d++) g_dx[d] = -g_Q * av[d];
   g.m_n = g_mpv * Msum;
   const double g_vis = g_mpv * N.m;
   g.p_n = g_vis * Vn * Vn;
@@ -488,56 +519,49 @@ __device__ inline void rhs_pair_adj(const RhsAdjConst &k, const double *etab, co
   g.rho_s = -2.0 * g_vis * PV2s * Vs;
   if (k.lans) {
     g_dotvv -= g_vis * k.eps_half * Vn;
-    g.rho_n += g_vis * k.eps_half * dotvv * Vn * Vn;
+    g.rho_n += g_vis * k.eps_half * q.dotvv * Vn * Vn;
   }
-  if (avis) {
-    const double g_mu = g_vis * (2.0 * k.beta * mu - k.alpha * cab) * irhoab;
-    g_cab -= g_vis * k.alpha * mu * irhoab;
-    const double g_rhoab = -g_vis * vis_a * irhoab;
+  if (q.avis) {
+    const double g_mu = g_vis * (2.0 * k.beta * q.mu - k.alpha * q.cab) * q.irhoab;
+    g_cab -= g_vis * k.alpha * q.mu * q.irhoab;
+    const double g_rhoab = -g_vis * q.vis_a * q.irhoab;
     g.rho_s += 0.5 * g_rhoab;
     g.rho_n += 0.5 * g_rhoab;
-    g_vrab += g_mu * h * iden;
-    g_r2 -= g_mu * mu * iden;
+    g_vrab += g_mu * q.h * q.iden;
+    g_r2 -= g_mu * q.mu * q.iden;
   }
   g.T_s = g.T_n = g.cp_s = g.cp_n = 0;
-  if (k.lans) {
+  if (q.heat) {
     // heat = base cp_ab |v|^2 Gd (T_s - T_n), Gd = G r^2 = dW.dx,
     // base = 10 eps / Pr m_n V_n V_s / (sum|v_k dx_k| h c_ab)
-    double dotvr = 0;
-    for (int q = 0; q < 3; q++) dotvr += fabs(vij[q] * dx[q]);
-    const double Gd = G * r2;
-    if (dotvr != 0 && Gd != 0) {
-      const double cpab = 0.5 * (S.cp + N.cp);
-      const double inv = rcp64(dotvr * h * cab);
-      const double base = (10.0 * k.eps_half / k.prandtl) * N.m * Vn * Vs * inv;
-      const double B = base * cpab * dotvv * Gd;
-      const double dT = S.T - N.T;
-      const double hh = ae * B * dT;
-      const double aeb = ae * dT * base;
-      g.T_s = ae * B;
-      g.T_n = -ae * B;
-      g.m_n += hh * N.im;
-      g.rho_n -= hh * Vn;
-      g.rho_s -= hh * Vs;
-      g.cp_s = g.cp_n = 0.5 * aeb * dotvv * Gd;
-      g_dotvv += aeb * cpab * Gd;
-      const double g_Gd = aeb * cpab * dotvv;
-      g_G += g_Gd * r2;
-      g_r2 += g_Gd * G;
-      const double g_dotvr = -hh * (inv * h * cab);
-      for (int q = 0; q < 3; q++) {
-        const double pr = vij[q] * dx[q];
-        const double sg = pr > 0 ? 1.0 : (pr < 0 ? -1.0 : 0.0);
-        g_vij[q] += g_dotvr * sg * dx[q];
-        g_dx[q] += g_dotvr * sg * vij[q];
-      }
-      g_cab -= hh * (inv * dotvr * h);
+    const double base = (10.0 * k.eps_half / k.prandtl) * N.m * Vn * Vs * q.hinv;
+    const double B = base * q.cpab * q.dotvv * q.Gd;
+    const double dT = S.T - N.T;
+    const double hh = ae * B * dT;
+    const double aeb = ae * dT * base;
+    g.T_s = ae * B;
+    g.T_n = -ae * B;
+    g.m_n += hh * N.im;
+    g.rho_n -= hh * Vn;
+    g.rho_s -= hh * Vs;
+    g.cp_s = g.cp_n = 0.5 * aeb * q.dotvv * q.Gd;
+    g_dotvv += aeb * q.cpab * q.Gd;
+    const double g_Gd = aeb * q.cpab * q.dotvv;
+    g_G += g_Gd * q.r2;
+    g_r2 += g_Gd * q.G;
+    const double g_dotvr = -hh * (q.hinv * q.h * q.cab);
+    for (int d = 0; d < 3; d++) {
+      const double pr = vij[d] * dx[d];
+      const double sg = pr > 0 ? 1.0 : (pr < 0 ? -1.0 : 0.0);
+      g_vij[d] += g_dotvr * sg * dx[d];
+      g_dx[d] += g_dotvr * sg * vij[d];
     }
+    g_cab -= hh * (q.hinv * q.dotvr * q.h);
   }
-  g_r2 -= g_G * G * ih2;
-  for (int q = 0; q < 3; q++) {
-    g.dx[q] = g_dx[q] + 2.0 * dx[q] * g_r2 + g_vrab * vij[q];
-    g.vij[q] = g_vij[q] + g_vrab * dx[q] + 2.0 * vij[q] * g_dotvv;
+  g_r2 -= g_G * q.G * q.ih2;
+  for (int d = 0; d < 3; d++) {
+    g.dx[d] = g_dx[d] + 2.0 * dx[d] * g_r2 + g_vrab * vij[d];
+    g.vij[d] = g_vij[d] + g_vrab * dx[d] + 2.0 * vij[d] * g_dotvv;
   }
   g.c_s = g.c_n = 0.5 * g_cab;
 }
@@ -576,20 +600,25 @@ struct RhsAdj {
   __device__ static double hker_nbr(double hi, double hj) { return 0.5 * (hj + hi); }
   __device__ static bool own_ok(const Row &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   __device__ static bool nbr_ok(const Row &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
-  __device__ static void own(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double,
-                             double hi, double hj) {
+  // dx = x_i - x_j (row minus candidate); the neighbour role sees -dx, -v_ij
+  __device__ static void pair(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double,
+                              double hi, double hj, bool fo, bool fn) {
+    const RhsSide J = side(s, q);
+    const double vij[3] = {t.me.vel[0] - J.vel[0], t.me.vel[1] - J.vel[1], t.me.vel[2] - J.vel[2]};
+    const RhsGeo G = rhs_geo(A.k, s.etab, dx, vij, 0.5 * (hi + hj), t.me, J);
     RhsGrad g;
-    rhs_pair_adj(A.k, s.etab, dx, 0.5 * (hi + hj), t.me, side(s, q), t.a, t.ae, g);
-    for (int k = 0; k < 3; k++) { t.g[k] += g.dx[k]; t.gu[k] += g.vij[k]; }
-    t.grho += g.rho_s; t.gp += g.p_s; t.gc += g.c_s; t.gT += g.T_s; t.gcp += g.cp_s;
-  }
-  __device__ static void nbr(const Args &A, Row &t, const Tile &s, uint32_t q, const double *dx, double,
-                             double hi, double hj) {
-    RhsGrad g;
-    const double a[3] = {s.a[0][q], s.a[1][q], s.a[2][q]};
-    rhs_pair_adj(A.k, s.etab, dx, 0.5 * (hj + hi), side(s, q), t.me, a, s.ae[q], g);
-    for (int k = 0; k < 3; k++) { t.g[k] -= g.dx[k]; t.gu[k] -= g.vij[k]; }
-    t.gm += g.m_n; t.grho += g.rho_n; t.gp += g.p_n; t.gc += g.c_n; t.gT += g.T_n; t.gcp += g.cp_n;
+    if (fo) {
+      rhs_role(A.k, G, dx, vij, t.me, J, t.a, t.ae, g);
+      for (int k = 0; k < 3; k++) { t.g[k] += g.dx[k]; t.gu[k] += g.vij[k]; }
+      t.grho += g.rho_s; t.gp += g.p_s; t.gc += g.c_s; t.gT += g.T_s; t.gcp += g.cp_s;
+    }
+    if (fn) {
+      const double dn[3] = {-dx[0], -dx[1], -dx[2]}, vn[3] = {-vij[0], -vij[1], -vij[2]};
+      const double a[3] = {s.a[0][q], s.a[1][q], s.a[2][q]};
+      rhs_role(A.k, G, dn, vn, J, t.me, a, s.ae[q], g);
+      for (int k = 0; k < 3; k++) { t.g[k] -= g.dx[k]; t.gu[k] -= g.vij[k]; }
+      t.gm += g.m_n; t.grho += g.rho_n; t.gp += g.p_n; t.gc += g.c_n; t.gT += g.T_n; t.gcp += g.cp_n;
+    }
   }
   __device__ static void combine(Row &t) {
     for (int k = 0; k < 3; k++) { t.g[k] = sub_sum<kSub>(t.g[k]); t.gu[k] = sub_sum<kSub>(t.gu[k]); }

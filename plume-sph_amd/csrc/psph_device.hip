@@ -115,27 +115,42 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// dt candidate min h/c over need_neigh particles (timestep.cc:94-135).
-__global__ void k_dt_min(uint32_t n, const int32_t *bc, const double *h, const double *c,
-                         unsigned long long *out, int *err) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Block-wide min / max of one value per thread (blockDim = 256); the
+// result is valid in thread 0.  One atomic per block keeps the global
+// update off a single contended address.
+template <bool MAX>
+__device__ inline double block_ext(double v) {
+  __shared__ double part[4];
+  v = MAX ? wave_max(v) : wave_min(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < 4; w++) v = MAX ? fmax(v, part[w]) : fmin(v, part[w]);
+  return v;
+}
+
+// dt candidate min h/c over need_neigh particles (timestep.cc:94-135);
+// grid-stride, 256 threads per block.
+__global__ __launch_bounds__(256) void k_dt_min(uint32_t n, const int32_t *bc, const double *h, const double *c,
+                                                unsigned long long *out, int *err) {
   double v = 1.0e300;
-  if (i < n && need_neigh(bc[i])) {
-    double ci = c[i];
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    if (!need_neigh(bc[i])) continue;
+    const double ci = c[i];
     if (!(ci > 0)) raise_err(err, PSPH_EINVARIANT, i);
-    else v = h[i] / ci;
+    else v = fmin(v, h[i] / ci);
   }
-  v = wave_min(v);
-  if ((threadIdx.x & 63) == 0 && v < 1.0e300) atomicMin(out, (unsigned long long)__double_as_longlong(v));
+  v = block_ext<false>(v);
+  if (threadIdx.x == 0 && v < 1.0e300) atomicMin(out, (unsigned long long)__double_as_longlong(v));
 }
 
 // max h over live particles (the cell edge must cover every pair support)
-__global__ void k_h_max(uint32_t n, const int32_t *bc, const double *h, unsigned long long *out) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_h_max(uint32_t n, const int32_t *bc, const double *h, unsigned long long *out) {
   double v = 0;
-  if (i < n && bc[i] >= 0) v = h[i];
-  v = wave_max(v);
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    if (bc[i] >= 0) v = fmax(v, h[i]);
+  v = block_ext<true>(v);
+  if (threadIdx.x == 0 && v > 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 
 // Cell grid of the neighbour search.  Particles are sorted by
@@ -1151,6 +1166,8 @@ __global__ void k_conservation(uint32_t n, const int32_t *bc, const double *m, c
 }
 
 inline unsigned blocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+// grid of a grid-stride reduction: enough blocks to fill the chip, no more
+inline unsigned red_blocks(size_t n) { return std::min(blocks(n), 2048u); }
 
 }  // namespace
 
@@ -1470,7 +1487,7 @@ extern "C" int psph_timestep(psph_ctx *c, double *dt_out) {
   unsigned long long init = (unsigned long long)0x7FF0000000000000ull;   // +inf bits
   StageTimer tm(c, 0);
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64, init, 1);
-  if (c->n) hipLaunchKernelGGL(k_dt_min, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->f.c, c->d_u64, c->d_err);
+  if (c->n) hipLaunchKernelGGL(k_dt_min, red_blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->f.c, c->d_u64, c->d_err);
   tm.stop();
   unsigned long long bits;
   HIP_TRY(hipMemcpyAsync(&bits, c->d_u64, 8, hipMemcpyDeviceToHost, c->stream));
@@ -1496,7 +1513,7 @@ static int build_grid(psph_ctx *c) {
   const psph_config &k = c->cfg;
   // h_max over live particles
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 1, 0ull, 1);
-  if (c->n) hipLaunchKernelGGL(k_h_max, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->d_u64 + 1);
+  if (c->n) hipLaunchKernelGGL(k_h_max, red_blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->d_u64 + 1);
   unsigned long long bits;
   HIP_TRY(hipMemcpyAsync(&bits, c->d_u64 + 1, 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1694,7 +1711,7 @@ static EosConst eos_const(const psph_config &k) {
 
 static int reduce_hmax(psph_ctx *c, double *out) {
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 1, 0ull, 1);
-  if (c->n) hipLaunchKernelGGL(k_h_max, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->d_u64 + 1);
+  if (c->n) hipLaunchKernelGGL(k_h_max, red_blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, c->f.h, c->d_u64 + 1);
   unsigned long long bits;
   HIP_TRY(hipMemcpyAsync(&bits, c->d_u64 + 1, 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
