@@ -888,15 +888,25 @@ struct VsmArgs {
   double *o_wn;            // optional: the filter's normalisation sum (adjoint recompute)
 };
 
+// The FP32-pair form evaluates the filter weights in FP64 from the exp table
+// (kernel argument, exponential, m_j / rho_j): sv moves the particles, and
+// FP32 weights (relative error ~3e-7, dominated by the rounding of s^2 and of
+// h) made the velocity filter the largest source of FP32-vs-FP64 difference,
+// ~15x the RHS's FP32 terms.
+struct VsmMixTile { double mr64[kWave], v64[3][kWave], etab[kETab]; };
+template <class R> struct VsmTileR { R mr[kWave], vx[kWave], vy[kWave], vz[kWave]; };
+
 template <class R>
 struct VsmPass {
   using Args = VsmArgs;
-  template <class T> __device__ static void init(T &, int) {}
-  struct Tile {
-    R mr[kWave], vx[kWave], vy[kWave], vz[kWave];
+  static constexpr bool kMixed = std::is_same<R, float>::value;
+  struct Tile : std::conditional<kMixed, VsmMixTile, VsmTileR<R>>::type {
     int ok[kWave];
   };
-  struct Tgt { double hi, supp, v64[3], dv[3], wn; R ih, norm, v[3]; };
+  __device__ static void init(Tile &s, int lane) {
+    if constexpr (kMixed) etab_fill(s.etab, lane);
+  }
+  struct Tgt { double hi, supp, ih2, nrm, v64[3], dv[3], wn; R ih, norm, v[3]; };
   static constexpr int kSub = ::kSub;
   __device__ static void combine(Tgt &t) {
     for (int k = 0; k < 3; k++) t.dv[k] = sub_sum<kSub>(t.dv[k]);
@@ -911,16 +921,22 @@ struct VsmPass {
     const double ih = 1.0 / t.hi;
     const double nk = kInvSqrtPi * ih;
     t.ih = (R)ih;
-    t.norm = (R)(a.vy ? nk * nk * nk : nk);
+    t.ih2 = ih * ih;
+    t.nrm = a.vy ? nk * nk * nk : nk;
+    t.norm = (R)t.nrm;
     t.v64[0] = a.vx[i]; t.v64[1] = a.vy ? a.vy[i] : 0; t.v64[2] = a.vz ? a.vz[i] : 0;
     for (int k = 0; k < 3; k++) t.v[k] = (R)t.v64[k];
     t.dv[0] = t.dv[1] = t.dv[2] = 0; t.wn = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    s.mr[l] = (R)(a.m[j] / a.rho[j]);
-    s.vx[l] = (R)a.vx[j];
-    s.vy[l] = a.vy ? (R)a.vy[j] : R(0);
-    s.vz[l] = a.vz ? (R)a.vz[j] : R(0);
+    const double v[3] = {a.vx[j], a.vy ? a.vy[j] : 0.0, a.vz ? a.vz[j] : 0.0};
+    if constexpr (kMixed) {
+      s.mr64[l] = a.m[j] / a.rho[j];
+      for (int k = 0; k < 3; k++) s.v64[k][l] = v[k];
+    } else {
+      s.mr[l] = (R)(a.m[j] / a.rho[j]);
+      s.vx[l] = (R)v[0]; s.vy[l] = (R)v[1]; s.vz[l] = (R)v[2];
+    }
     s.ok[l] = contr_vel(a.bc[j]);
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
@@ -928,19 +944,30 @@ struct VsmPass {
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds) {
-    R expt = 0;
+    if constexpr (kMixed) {
+      double r2 = 0;
 #pragma unroll
-    for (int k = 0; k < D; k++) {
-      R sk = (R)ds[k] * t.ih;
-      expt += sk * sk;
+      for (int k = 0; k < D; k++) r2 = fma(ds[k], ds[k], r2);
+      const double w = t.nrm * exp_neg_tab(r2 * t.ih2, s.etab) * s.mr64[q];
+#pragma unroll
+      for (int k = 0; k < D; k++) t.dv[k] = fma(w, s.v64[k][q] - t.v64[k], t.dv[k]);
+      t.wn += w;
     }
-    const R mvw = t.norm * rexp<R>(-expt) * s.mr[q];
-    t.dv[0] += (double)(mvw * (s.vx[q] - t.v[0]));
-    if (D == 3) {
-      t.dv[1] += (double)(mvw * (s.vy[q] - t.v[1]));
-      t.dv[2] += (double)(mvw * (s.vz[q] - t.v[2]));
+    else {
+      R expt = 0;
+#pragma unroll
+      for (int k = 0; k < D; k++) {
+        R sk = (R)ds[k] * t.ih;
+        expt += sk * sk;
+      }
+      const R mvw = t.norm * rexp<R>(-expt) * s.mr[q];
+      t.dv[0] += (double)(mvw * (s.vx[q] - t.v[0]));
+      if (D == 3) {
+        t.dv[1] += (double)(mvw * (s.vy[q] - t.v[1]));
+        t.dv[2] += (double)(mvw * (s.vz[q] - t.v[2]));
+      }
+      t.wn += (double)mvw;
     }
-    t.wn += (double)mvw;
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
@@ -1742,7 +1769,10 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
     const double reach = std::min(k.cutoff * k.c_smooth * hmax, list_reach) * (1 + 1e-9);
     {
       StageTimer tm(c, 2);
-      if (c->n) launch_form<AdaptPass>(c, make_common(c, reach), a);
+      // FP64 in both forms: h enters every later pass, and a relative error
+      // of h moves every kernel value by ~|s|^2 times as much (two launches
+      // every SML_UPDATE_INT steps)
+      if (c->n) launch_pairs<AdaptPass<double>>(c, make_common(c, reach), a);
     }
     HIP_TRY(hipGetLastError());
     PSPH_TRY(reduce_hmax(c, &hmax));
