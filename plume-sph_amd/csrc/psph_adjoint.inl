@@ -179,6 +179,7 @@ struct AdjCommon {
 };
 
 // ROLES: 1 own role only, 2 neighbour role only, 3 both (one traversal).
+static_assert(kWave / kSub <= 32, "the adjoint's role masks hold one bit per candidate of a lane's share");
 template <class P, int ROLES>
 __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], slf[kWave];
@@ -231,28 +232,37 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
         __syncthreads();
         const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
         if (active) {
-          uint64_t mo = 0, mn = 0;
-          for (uint32_t q = sub; q < cnt; q += kSub) {
-            const double d0 = xi[0] - sx[q], d1 = xi[1] - sy[q], d2 = xi[2] - sz[q];
+          // both roles' predicates, without branches; bit b <-> q = sub + b kSub
+          uint32_t mo = 0, mn = 0;
+          const uint32_t nb = (cnt + kSub - 1) / kSub;
+#pragma unroll 2
+          for (uint32_t b = 0; b < nb; b++) {
+            const uint32_t q = sub + b * kSub;
+            const bool valid = q < cnt;
+            const uint32_t qq = valid ? q : 0u;
+            const double d0 = xi[0] - sx[qq], d1 = xi[1] - sy[qq], d2 = xi[2] - sz[qq];
             const double amax = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
-            const bool same = pc.list_all || srefb[q] == rb_i;
-            const uint32_t jq = base + q;
-            if ((ROLES & 1) && tgt_i && P::own_ok(t, tile, q, jq, i) && (same || amax <= lf_i) &&
-                amax <= supp_i && !pc.cut.outside(amax, P::hker_own(hi, sh[q])))
-              mo |= 1ull << q;
-            if ((ROLES & 2) && need_neigh(sbc[q]) && P::nbr_ok(t, tile, q, jq, i) && (same || amax <= slf[q]) &&
-                amax <= pc.cutoff2 * sh[q] && !pc.cut.outside(amax, P::hker_nbr(hi, sh[q])))
-              mn |= 1ull << q;
+            const bool same = (pc.list_all != 0) | (srefb[qq] == rb_i);
+            const uint32_t jq = base + qq;
+            bool own = valid & ((ROLES & 1) != 0) & tgt_i;
+            own &= P::own_ok(t, tile, qq, jq, i) & (same | (amax <= lf_i)) & (amax <= supp_i);
+            own &= !pc.cut.outside(amax, P::hker_own(hi, sh[qq]));
+            bool nbr = valid & ((ROLES & 2) != 0) & need_neigh(sbc[qq]);
+            nbr &= P::nbr_ok(t, tile, qq, jq, i) & (same | (amax <= slf[qq])) & (amax <= pc.cutoff2 * sh[qq]);
+            nbr &= !pc.cut.outside(amax, P::hker_nbr(hi, sh[qq]));
+            mo |= (uint32_t)own << b;
+            mn |= (uint32_t)nbr << b;
           }
           // one evaluation per candidate met in either role: the pair's
           // kernel and viscosity geometry is shared by both roles
-          uint64_t mb = mo | mn;
+          uint32_t mb = mo | mn;
           while (mb) {
-            const uint32_t q = (uint32_t)__builtin_ctzll(mb);
+            const uint32_t b = (uint32_t)__builtin_ctz(mb);
             mb &= mb - 1;
+            const uint32_t q = sub + b * kSub;
             const double dx[3] = {xi[0] - sx[q], xi[1] - sy[q], xi[2] - sz[q]};
             const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-            P::pair(a, t, tile, q, dx, amax, hi, sh[q], (mo >> q) & 1, (mn >> q) & 1);
+            P::pair(a, t, tile, q, dx, amax, hi, sh[q], (mo >> b) & 1, (mn >> b) & 1);
           }
         }
         __syncthreads();

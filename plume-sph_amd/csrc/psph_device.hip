@@ -382,6 +382,7 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   const int lane = threadIdx.x;
   const uint32_t slot = (uint32_t)lane / P::kSub, sub = (uint32_t)lane % P::kSub;
   const bool active = slot < cc;
+  using Mask = typename std::conditional<(kWave / P::kSub > 32), uint64_t, uint32_t>::type;
   const uint32_t i = pc.targets[cf + (active ? slot : 0)];
 
   double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
@@ -413,34 +414,32 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
             } else {
               // empty slot: a position no predicate accepts (|dx| > every support)
               sx[lane] = sy[lane] = sz[lane] = kFar;
+              srefb[lane] = ~0u;
+              sgh[lane] = 0;
             }
             __syncthreads();
             const uint32_t cnt = min((uint32_t)kWave, hi - base);
             if (active && P::want(t)) {
-              // phase 1: exact FP64 predicate on this lane's share of the tile
-              // (the same trip count on every lane; q < 64 always)
-              uint64_t mask = 0;
+              // phase 1: exact FP64 predicate on this lane's share of the tile,
+              // without branches (the same trip count on every lane, q < 64)
+              Mask mask = 0;
               const uint32_t nb = (cnt + P::kSub - 1) / P::kSub;
+#pragma unroll 4
               for (uint32_t b = 0; b < nb; b++) {
                 const uint32_t q = sub + b * P::kSub;
-                double dx[3];
-                dx[0] = xi[0] - sx[q];
-                double amax = fabs(dx[0]);
-                if (D == 3) {
-                  dx[1] = xi[1] - sy[q];
-                  dx[2] = xi[2] - sz[q];
-                  amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
-                }
-                // outside the kernel box (exact, see CutTest)
-                if (pc.cut.outside(amax, P::hker(a, t, tile, q))) continue;
-                // list membership (neighs.cc:37-211): same bucket or 4.25 h_i box
-                const bool in = (pc.list_all || srefb[q] == rb_i || amax <= lf_i) &&
-                                amax <= t.supp && P::extra(a, t, tile, q, base + q, i);
-                if (in) mask |= 1ull << b;
+                double amax = fabs(xi[0] - sx[q]);
+                if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
+                // inside the kernel box (exact, see CutTest), list membership
+                // (neighs.cc:37-211: same bucket or 4.25 h_i box), in_support
+                // and the pass's own mask
+                bool in = !pc.cut.outside(amax, P::hker(a, t, tile, q));
+                in &= (pc.list_all != 0) | (srefb[q] == rb_i) | (amax <= lf_i);
+                in &= (q < cnt) & (amax <= t.supp) & P::extra(a, t, tile, q, base + q, i);
+                mask |= (Mask)in << b;
               }
               // phase 2: each lane walks its own hits
               while (mask) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(mask);
+                const uint32_t b = (uint32_t)__builtin_ctzll((uint64_t)mask);
                 mask &= mask - 1;
                 const uint32_t q = sub + b * P::kSub;
                 double dx[3];
