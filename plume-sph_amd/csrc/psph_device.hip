@@ -458,6 +458,183 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   if (active && sub == 0) P::finish(a, t, i, pc.err);
 }
 
+// ---------------------------------------------------------------------------
+// Visit lists.  The RHS, density and filter passes of a step see the same
+// positions, smoothing lengths and list-time data, so the exact candidate
+// scan is done once per step: for every chunk, the tiles that meet its grown
+// box and hold a pair of any of the three passes, each with one 32-bit hit
+// mask per lane (bit b <-> candidate sub + b kSub of the lane's target).  The
+// mask is a superset: list membership and in_support are exact, the kernel
+// box is widened to cut * max(h_ij, h_i) (the union of the three passes'
+// boxes) with a relative margin of 1e-12; each pass re-tests its own exact box
+// and masks on the set bits only.
+// ---------------------------------------------------------------------------
+static_assert(kWave / kSub <= 32, "visit masks hold one bit per candidate of a lane's share");
+
+struct VisitList {
+  const uint32_t *off;      // first visit of each chunk
+  const uint32_t *num;      // visits written for each chunk
+  const uint2 *ent;         // (first particle, particle count) of the visited tile
+  const uint32_t *mask;     // per visit, kWave lane masks
+};
+
+// tiles that meet each chunk's grown box (an upper bound on its visits);
+// one wavefront per chunk, chunk = blockIdx.x, zero past the last chunk
+__global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *vcount) {
+  const uint32_t chunk = blockIdx.x;
+  const uint32_t nch = *pc.nchunks;
+  if (chunk >= nch) {
+    if (threadIdx.x == 0) vcount[chunk] = 0;
+    return;
+  }
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
+  const uint32_t slot = (uint32_t)threadIdx.x / kSub;
+  const uint32_t i = pc.targets[cf + (slot < cc ? slot : 0)];
+  const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  const ChunkBox box = chunk_box(pc.tr, xi);
+  const GridDesc &g = pc.tr.g;
+  uint32_t cnt = 0;
+  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
+    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
+      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
+        const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+        const uint32_t ntile = pc.tr.toff[cell + 1] - pc.tr.toff[cell];
+        for (uint32_t t = 0; t < ntile; t++) cnt += tile_meets(box, pc.tr.tbox, pc.tr.toff[cell] + t) ? 1u : 0u;
+      }
+  if (threadIdx.x == 0) vcount[chunk] = cnt;
+}
+
+// the hit masks; every lane writes its mask of a visited tile, lane 0 the entry
+template <int D>
+__global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const double *h, double cutoff2, double cutx,
+                                                       const uint32_t *voff, uint32_t *vnum, uint2 *vent,
+                                                       uint32_t *vmask) {
+  __shared__ double sx[kWave], sy[kWave], sz[kWave], shh[kWave];
+  __shared__ uint32_t srefb[kWave];
+  const uint32_t nch = *pc.nchunks;
+  const uint32_t per = (nch + kXcds - 1) / kXcds;
+  const uint32_t local = blockIdx.x / kXcds;
+  if (local >= per) return;
+  const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
+  if (chunk >= nch) return;
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
+  const int lane = threadIdx.x;
+  const uint32_t slot = (uint32_t)lane / kSub, sub = (uint32_t)lane % kSub;
+  const bool active = slot < cc;
+  const uint32_t i = pc.targets[cf + (active ? slot : 0)];
+  const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  const uint32_t rb_i = pc.refb[i];
+  const double hi = h[i], hh_i = 0.5 * hi;
+  const double lf_i = pc.list_factor * pc.h_list[i], supp_i = cutoff2 * hi;
+  const ChunkBox box = chunk_box(pc.tr, xi);
+  const GridDesc &g = pc.tr.g;
+  const uint32_t w0 = voff[chunk];
+  uint32_t w = w0;
+  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
+    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
+      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
+        const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+        const uint32_t lo = pc.tr.cs[cell], hi_ = pc.tr.cs[cell + 1];
+        uint32_t tl = pc.tr.toff[cell];
+        for (uint32_t base = lo; base < hi_; base += kWave, tl++) {
+          if (!tile_meets(box, pc.tr.tbox, tl)) continue;
+          const uint32_t j = base + lane;
+          if (j < hi_) {
+            sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+            shh[lane] = 0.5 * h[j];
+            srefb[lane] = pc.refb[j];
+          } else {
+            sx[lane] = sy[lane] = sz[lane] = kFar;
+            shh[lane] = 0;
+            srefb[lane] = ~0u;
+          }
+          __syncthreads();
+          const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
+          uint32_t mask = 0;
+          if (active) {
+            const uint32_t nb = (cnt + kSub - 1) / kSub;
+#pragma unroll 4
+            for (uint32_t b = 0; b < nb; b++) {
+              const uint32_t q = sub + b * kSub;
+              double amax = fabs(xi[0] - sx[q]);
+              if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
+              bool in = amax <= cutx * fmax(hh_i + shh[q], hi);
+              in &= (pc.list_all != 0) | (srefb[q] == rb_i) | (amax <= lf_i);
+              in &= amax <= supp_i;
+              mask |= (uint32_t)in << b;
+            }
+          }
+          if (__ballot(mask != 0)) {
+            if (lane == 0) vent[w] = make_uint2(base, cnt);
+            vmask[(size_t)w * kWave + lane] = mask;
+            w++;
+          }
+          __syncthreads();
+        }
+      }
+  if (lane == 0) vnum[chunk] = w - w0;
+}
+
+// A pair pass over the visit lists: stage each visited tile, test the pass's
+// exact kernel box and masks on the set bits, and run the pair arithmetic.
+template <int D, class P>
+__global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, typename P::Args a) {
+  __shared__ double sx[kWave], sy[kWave], sz[kWave];
+  __shared__ int sgh[kWave];
+  __shared__ typename P::Tile tile;
+  const uint32_t nch = *pc.nchunks;
+  const uint32_t per = (nch + kXcds - 1) / kXcds;
+  const uint32_t local = blockIdx.x / kXcds;
+  if (local >= per) return;
+  const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
+  if (chunk >= nch) return;
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
+  const int lane = threadIdx.x;
+  const uint32_t slot = (uint32_t)lane / P::kSub, sub = (uint32_t)lane % P::kSub;
+  const bool active = slot < cc;
+  const uint32_t i = pc.targets[cf + (active ? slot : 0)];
+  const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  typename P::Tgt t;
+  P::load_target(a, i, t);
+  P::init(tile, lane);
+  __syncthreads();
+  P::begin_pass(a, t, 0);
+  const uint32_t v0 = vl.off[chunk], nv = vl.num[chunk];
+  for (uint32_t v = 0; v < nv; v++) {
+    const uint2 e = vl.ent[v0 + v];
+    const uint32_t base = e.x, cnt = e.y;
+    if ((uint32_t)lane < cnt) {
+      const uint32_t j = base + lane;
+      sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+      if (pc.gflag) { const int b = pc.bc[j]; sgh[lane] = b == 1 || b == 2; }
+      P::stage(a, j, tile, lane);
+    }
+    uint32_t mask = vl.mask[(size_t)(v0 + v) * kWave + lane];
+    __syncthreads();
+    if (active) {
+      while (mask) {
+        const uint32_t b = (uint32_t)__builtin_ctz(mask);
+        mask &= mask - 1;
+        const uint32_t q = sub + b * P::kSub;
+        double dx[3];
+        dx[0] = xi[0] - sx[q];
+        double amax = fabs(dx[0]);
+        if (D == 3) {
+          dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q];
+          amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
+        }
+        if (pc.cut.outside(amax, P::hker(a, t, tile, q)) || !P::extra(a, t, tile, q, base + q, i)) continue;
+        P::pair(a, t, tile, q, base + q, i, dx);
+        if (pc.gflag && sgh[q]) pc.gflag[base + q] = 1u;
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (P::kSub > 1) P::combine(t);
+  P::end_pass(a, t, 0, i, active);
+  if (active && sub == 0) P::finish(a, t, i, pc.err);
+}
+
 // sum of v over the kSub lanes of a target (lanes slot*kSub .. +kSub-1)
 template <int S>
 __device__ inline double sub_sum(double v) {
@@ -1282,6 +1459,11 @@ struct psph_ctx {
   bool grid_valid = false;
   GridDesc grid{};
   double reach_main = 0;       // per-axis reach of the RHS / density / filter predicates
+  // visit lists of the RHS / density / filter passes (built once per step)
+  bool vis_valid = false;
+  uint32_t *vcount = nullptr, *voff = nullptr, *vnum = nullptr, *vmask = nullptr;
+  uint2 *vent = nullptr;
+  size_t vchunk_cap = 0, vis_cap = 0;
   double h_max = 0;
   // clock (TimeProps)
   double time = 0, next_output = 0;
@@ -1417,7 +1599,8 @@ extern "C" int psph_destroy(psph_ctx *c) {
   for (auto p : c->dsp) if (p) hipFree(p);
   void *ptrs[] = {c->usp, c->isp, c->h_list, c->pid, c->pid_sp, c->refb, c->key_a, c->key_b, c->idx_a, c->idx_b,
                   c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->cs, c->tcnt, c->toff,
-                  c->cchunks, c->coff, c->tbox, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part};
+                  c->cchunks, c->coff, c->tbox, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part,
+                  c->vcount, c->voff, c->vnum, c->vmask, c->vent};
   for (auto p : ptrs) if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -1677,6 +1860,7 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   HIP_TRY(hipGetLastError());
   tm.stop();
   c->grid_valid = true;
+  c->vis_valid = false;
   return 0;
 }
 
@@ -1731,6 +1915,67 @@ static void launch_form(psph_ctx *c, const PairCommon &pc, const typename P<doub
   else launch_pairs<P<double>>(c, pc, a);
 }
 
+// Visit lists of the current step (positions, h and targets fixed since the
+// last psph_neighbors / adaptation): count the tiles each chunk meets, scan,
+// then write the tiles with hits and their lane masks.
+static int ensure_visits(psph_ctx *c) {
+  if (c->vis_valid) return 0;
+  StageTimer tm(c, 1);
+  const PairCommon pc = make_common(c, c->reach_main);
+  const unsigned G = chunk_grid(c->n, c->grid.ncells, kChunk);
+  if ((size_t)G + 1 > c->vchunk_cap) {
+    uint32_t **arrs[] = {&c->vcount, &c->voff, &c->vnum};
+    for (auto pp : arrs) { if (*pp) hipFree(*pp); *pp = nullptr; }
+    for (auto pp : arrs) PSPH_TRY(dmalloc((void **)pp, ((size_t)G + 1) * 4));
+    c->vchunk_cap = (size_t)G + 1;
+  }
+  hipLaunchKernelGGL(k_visit_count, G, kWave, 0, c->stream, pc, c->vcount);
+  hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, c->stream, (int *)c->vcount + G, 0, 1);
+  HIP_TRY(hipGetLastError());
+  PSPH_TRY(excl_scan(c, c->vcount, c->voff, (size_t)G + 1));
+  uint32_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, c->voff + G, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if ((size_t)total > c->vis_cap) {
+    if (c->vmask) hipFree(c->vmask);
+    if (c->vent) hipFree(c->vent);
+    c->vmask = nullptr; c->vent = nullptr;
+    const size_t cap = (size_t)total + total / 8 + 1024;
+    PSPH_TRY(dmalloc((void **)&c->vmask, cap * kWave * 4));
+    PSPH_TRY(dmalloc((void **)&c->vent, cap * sizeof(uint2)));
+    c->vis_cap = cap;
+  }
+  // a superset of the three passes' kernel boxes with a relative margin
+  const double cutx = c->cfg.cutoff * (1.0 + 1e-12);
+  if (c->cfg.dim == 3)
+    hipLaunchKernelGGL(k_visit_masks<3>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->voff,
+                       c->vnum, c->vent, c->vmask);
+  else
+    hipLaunchKernelGGL(k_visit_masks<1>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->voff,
+                       c->vnum, c->vent, c->vmask);
+  HIP_TRY(hipGetLastError());
+  c->vis_valid = true;
+  return 0;
+}
+
+// the RHS, density and filter passes run over the visit lists
+template <template <class> class P>
+static int launch_visits(psph_ctx *c, const typename P<double>::Args &a) {
+  PSPH_TRY(ensure_visits(c));
+  const PairCommon pc = make_common(c, c->reach_main);
+  const VisitList vl{c->voff, c->vnum, c->vent, c->vmask};
+  const unsigned nb = chunk_grid(c->n, c->grid.ncells, kChunk);
+  const bool f32 = c->cfg.precision == PSPH_FP32_PAIR;
+  if (c->cfg.dim == 3) {
+    if (f32) hipLaunchKernelGGL((k_pairs_v<3, Bound<3, P<float>>>), nb, kWave, 0, c->stream, pc, vl, a);
+    else hipLaunchKernelGGL((k_pairs_v<3, Bound<3, P<double>>>), nb, kWave, 0, c->stream, pc, vl, a);
+  } else {
+    if (f32) hipLaunchKernelGGL((k_pairs_v<1, Bound<1, P<float>>>), nb, kWave, 0, c->stream, pc, vl, a);
+    else hipLaunchKernelGGL((k_pairs_v<1, Bound<1, P<double>>>), nb, kWave, 0, c->stream, pc, vl, a);
+  }
+  return 0;
+}
+
 static EosConst eos_const(const psph_config &k) {
   return EosConst{k.ng0, k.cvs, k.cvg, k.cva, k.rg, k.ra, k.lans};
 }
@@ -1747,6 +1992,7 @@ static int reduce_hmax(psph_ctx *c, double *out) {
 
 // the later passes of a step use the new h: widen the reach
 static void set_reach(psph_ctx *c, double hmax) {
+  c->vis_valid = false;                // h changed
   c->h_max = hmax;
   c->reach_main = c->cfg.cutoff * hmax * (1 + 1e-9);
 }
@@ -1812,9 +2058,10 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   a.ata = k.ata; a.eps_half = k.eps_half; a.prandtl = k.prandtl; a.heat_scale = k.heat_scale;
   a.lans = k.lans; a.dim = k.dim; a.eos = eos_const(k); a.pairs = c->d_u64 + 2;
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 2, 0ull, 1);
+  if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 3);
-    if (n) launch_form<RhsPass>(c, make_common(c, c->reach_main), a);
+    if (n) PSPH_TRY(launch_visits<RhsPass>(c, a));
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
@@ -1842,9 +2089,10 @@ extern "C" int psph_density(psph_ctx *c) {
   a.cut = cut_test(k.cutoff);
   a.o_wn = c->rec ? c->rec->d[RF_WNE] : nullptr;
   a.e_sq16 = k.e_smooth_ratio * k.e_smooth_ratio == 16.0;
+  if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 4);
-    if (n) launch_form<DensPass>(c, make_common(c, c->reach_main), a);
+    if (n) PSPH_TRY(launch_visits<DensPass>(c, a));
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.rho, c->dsp[1]);
@@ -1862,9 +2110,10 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   a.bc = c->f.bc; a.svx = c->f.svx; a.svy = k.dim == 3 ? c->f.svy : nullptr; a.svz = k.dim == 3 ? c->f.svz : nullptr;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2;
   a.o_wn = c->rec ? c->rec->d[RF_WNV] : nullptr;
+  if (c->n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 5);
-    if (c->n) launch_form<VsmPass>(c, make_common(c, c->reach_main), a);
+    if (c->n) PSPH_TRY(launch_visits<VsmPass>(c, a));
   }
   HIP_TRY(hipGetLastError());
   return check_err(c, "vsmooth");
