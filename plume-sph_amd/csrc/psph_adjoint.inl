@@ -160,13 +160,9 @@ __global__ void k_adj_rhs_pre(uint32_t n, const int32_t *bc, double dt, double g
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint pair skeleton.  Rows: the particles whose state can depend on the
-// inputs (real particles, erupt ghosts, boundary ghosts next to targets).  For row i and
-// candidate j it evaluates the forward pass's exact pair predicate twice:
-// with i as the summing particle (own role, i a target) and with j as the
-// summing particle (neighbour role, j a target), and calls pair() once with
-// both flags and dx = x_i - x_j.  Within a pass, dx given to an own()/nbr()
-// role routine is always summing minus neighbour.
+// Adjoint pair passes.  Rows: the particles whose state can depend on the
+// inputs (real particles, erupt ghosts, boundary ghosts next to targets),
+// in chunks of kChunk consecutive rows of one cell.
 // ---------------------------------------------------------------------------
 struct AdjCommon {
   const double *x, *y, *z, *h, *h_list;
@@ -178,13 +174,132 @@ struct AdjCommon {
   CutTest cut;
 };
 
-// ROLES: 1 own role only, 2 neighbour role only, 3 both (one traversal).
+// Visit lists of a reversed step, shared by its three adjoint passes: for
+// every chunk of rows, the tiles that meet its grown box and hold a candidate
+// of either role of any pass, with two 32-bit masks per lane (own role,
+// neighbour role; bit b <-> candidate sub + b kSub).  List membership,
+// in_support and the row/candidate target tests are exact; the kernel box is
+// widened to the union of the passes' boxes (cut * max(h_s, h_ij), h_s the
+// summing particle's h) with a relative margin of 1e-12.  Each pass re-tests
+// its own box and masks on the set bits.
 static_assert(kWave / kSub <= 32, "the adjoint's role masks hold one bit per candidate of a lane's share");
-template <class P, int ROLES>
-__global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::Args a) {
+
+struct AdjVisits {
+  const uint32_t *off, *num;
+  const uint2 *ent;          // (first particle, particle count) of the visited tile
+  const uint2 *mask;         // per visit and lane: (own-role mask, neighbour-role mask)
+};
+
+__global__ __launch_bounds__(kWave) void k_adj_visit_count(AdjCommon pc, uint32_t *vcount) {
+  const uint32_t chunk = blockIdx.x;
+  const uint32_t nch = *pc.nchunks;
+  if (chunk >= nch) {
+    if (threadIdx.x == 0) vcount[chunk] = 0;
+    return;
+  }
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
+  const uint32_t slot = (uint32_t)threadIdx.x / kSub;
+  const uint32_t i = pc.rows[cf + (slot < cc ? slot : 0)];
+  const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  const ChunkBox box = chunk_box(pc.tr, xi);
+  const GridDesc &g = pc.tr.g;
+  uint32_t cnt = 0;
+  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
+    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
+      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
+        const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+        const uint32_t ntile = pc.tr.toff[cell + 1] - pc.tr.toff[cell];
+        for (uint32_t t = 0; t < ntile; t++) cnt += tile_meets(box, pc.tr.tbox, pc.tr.toff[cell] + t) ? 1u : 0u;
+      }
+  if (threadIdx.x == 0) vcount[chunk] = cnt;
+}
+
+__global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double cutx, const uint32_t *voff,
+                                                           uint32_t *vnum, uint2 *vent, uint2 *vmask) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], slf[kWave];
   __shared__ uint32_t srefb[kWave];
   __shared__ int sbc[kWave];
+  const uint32_t nch = *pc.nchunks;
+  const uint32_t per = (nch + kXcds - 1) / kXcds;
+  const uint32_t local = blockIdx.x / kXcds;
+  if (local >= per) return;
+  const uint32_t chunk = (blockIdx.x % kXcds) * per + local;
+  if (chunk >= nch) return;
+  const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
+  const int lane = threadIdx.x;
+  const uint32_t slot = (uint32_t)lane / kSub, sub = (uint32_t)lane % kSub;
+  const bool active = slot < cc;
+  const uint32_t i = pc.rows[cf + (active ? slot : 0)];
+  const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  const uint32_t rb_i = pc.refb[i];
+  const double hi = pc.h[i];
+  const double lf_i = pc.list_factor * pc.h_list[i];
+  const double supp_i = pc.cutoff2 * hi;
+  const bool tgt_i = need_neigh(pc.bc[i]);
+  const ChunkBox box = chunk_box(pc.tr, xi);
+  const GridDesc &g = pc.tr.g;
+  const uint32_t w0 = voff[chunk];
+  uint32_t w = w0;
+  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
+    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
+      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
+        const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+        const uint32_t lo = pc.tr.cs[cell], hi_ = pc.tr.cs[cell + 1];
+        uint32_t tl = pc.tr.toff[cell];
+        for (uint32_t base = lo; base < hi_; base += kWave, tl++) {
+          if (!tile_meets(box, pc.tr.tbox, tl)) continue;
+          const uint32_t j = base + lane;
+          if (j < hi_) {
+            sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+            sh[lane] = pc.h[j];
+            slf[lane] = pc.list_factor * pc.h_list[j];
+            srefb[lane] = pc.refb[j];
+            sbc[lane] = pc.bc[j];
+          } else {
+            sx[lane] = sy[lane] = sz[lane] = kFar;
+            sh[lane] = slf[lane] = 0;
+            srefb[lane] = ~0u;
+            sbc[lane] = -1;
+          }
+          __syncthreads();
+          const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
+          uint32_t mo = 0, mn = 0;
+          if (active) {
+            const uint32_t nb = (cnt + kSub - 1) / kSub;
+#pragma unroll 2
+            for (uint32_t b = 0; b < nb; b++) {
+              const uint32_t q = sub + b * kSub;
+              const double amax = fmax(fabs(xi[0] - sx[q]), fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
+              const bool same = (pc.list_all != 0) | (srefb[q] == rb_i);
+              const double hj = sh[q], hij = 0.5 * (hi + hj);
+              bool own = tgt_i & (same | (amax <= lf_i)) & (amax <= supp_i);
+              own &= amax <= cutx * fmax(hi, hij);
+              bool nbr = need_neigh(sbc[q]) & (same | (amax <= slf[q])) & (amax <= pc.cutoff2 * hj);
+              nbr &= amax <= cutx * fmax(hj, hij);
+              mo |= (uint32_t)own << b;
+              mn |= (uint32_t)nbr << b;
+            }
+          }
+          if (__ballot((mo | mn) != 0)) {
+            if (lane == 0) vent[w] = make_uint2(base, cnt);
+            vmask[(size_t)w * kWave + lane] = make_uint2(mo, mn);
+            w++;
+          }
+          __syncthreads();
+        }
+      }
+  if (lane == 0) vnum[chunk] = w - w0;
+}
+
+// An adjoint pair pass over the visit lists.  For row i and candidate j it
+// evaluates the forward pass's exact pair predicate with i as the summing
+// particle (own role, i a target) and with j as the summing particle
+// (neighbour role, j a target), and calls pair() once with both flags and
+// dx = x_i - x_j.  Within a pass, dx given to an own()/nbr() role routine is
+// always summing minus neighbour.
+template <class P>
+__global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, AdjVisits vl, typename P::Args a) {
+  __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave];
   __shared__ typename P::Tile tile;
   const uint32_t nch = *pc.nchunks;
   const uint32_t per = (nch + kXcds - 1) / kXcds;
@@ -198,76 +313,41 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, typename P::A
   const uint32_t slot = (uint32_t)lane / kSub, sub = (uint32_t)lane % kSub;
   const bool active = slot < cc;
   const uint32_t i = pc.rows[cf + (active ? slot : 0)];
-
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
-  const uint32_t rb_i = pc.refb[i];
   const double hi = pc.h[i];
-  const double lf_i = pc.list_factor * pc.h_list[i];
-  const double supp_i = pc.cutoff2 * hi;
-  const bool tgt_i = need_neigh(pc.bc[i]);
   typename P::Row t;
   P::load_row(a, i, t);
-  const ChunkBox box = chunk_box(pc.tr, xi);
-  const GridDesc &g = pc.tr.g;
   P::init(tile, lane);
   __syncthreads();
-
-  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
-    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
-      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
-      const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
-      const uint32_t lo = pc.tr.cs[cell], hi_ = pc.tr.cs[cell + 1];
-      uint32_t tl = pc.tr.toff[cell];
-      for (uint32_t base = lo; base < hi_; base += kWave, tl++) {
-        if (!tile_meets(box, pc.tr.tbox, tl)) continue;
-        const uint32_t j = base + lane;
-        if (j < hi_) {
-          sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
-          sh[lane] = pc.h[j];
-          slf[lane] = pc.list_factor * pc.h_list[j];
-          srefb[lane] = pc.refb[j];
-          sbc[lane] = pc.bc[j];
-          P::stage(a, j, tile, lane);
-        }
-        __syncthreads();
-        const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
-        if (active) {
-          // both roles' predicates, without branches; bit b <-> q = sub + b kSub
-          uint32_t mo = 0, mn = 0;
-          const uint32_t nb = (cnt + kSub - 1) / kSub;
-#pragma unroll 2
-          for (uint32_t b = 0; b < nb; b++) {
-            const uint32_t q = sub + b * kSub;
-            const bool valid = q < cnt;
-            const uint32_t qq = valid ? q : 0u;
-            const double d0 = xi[0] - sx[qq], d1 = xi[1] - sy[qq], d2 = xi[2] - sz[qq];
-            const double amax = fmax(fabs(d0), fmax(fabs(d1), fabs(d2)));
-            const bool same = (pc.list_all != 0) | (srefb[qq] == rb_i);
-            const uint32_t jq = base + qq;
-            bool own = valid & ((ROLES & 1) != 0) & tgt_i;
-            own &= P::own_ok(t, tile, qq, jq, i) & (same | (amax <= lf_i)) & (amax <= supp_i);
-            own &= !pc.cut.outside(amax, P::hker_own(hi, sh[qq]));
-            bool nbr = valid & ((ROLES & 2) != 0) & need_neigh(sbc[qq]);
-            nbr &= P::nbr_ok(t, tile, qq, jq, i) & (same | (amax <= slf[qq])) & (amax <= pc.cutoff2 * sh[qq]);
-            nbr &= !pc.cut.outside(amax, P::hker_nbr(hi, sh[qq]));
-            mo |= (uint32_t)own << b;
-            mn |= (uint32_t)nbr << b;
-          }
-          // one evaluation per candidate met in either role: the pair's
-          // kernel and viscosity geometry is shared by both roles
-          uint32_t mb = mo | mn;
-          while (mb) {
-            const uint32_t b = (uint32_t)__builtin_ctz(mb);
-            mb &= mb - 1;
-            const uint32_t q = sub + b * kSub;
-            const double dx[3] = {xi[0] - sx[q], xi[1] - sy[q], xi[2] - sz[q]};
-            const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-            P::pair(a, t, tile, q, dx, amax, hi, sh[q], (mo >> b) & 1, (mn >> b) & 1);
-          }
-        }
-        __syncthreads();
+  const uint32_t v0 = vl.off[chunk], nv = vl.num[chunk];
+  for (uint32_t v = 0; v < nv; v++) {
+    const uint2 e = vl.ent[v0 + v];
+    const uint32_t base = e.x, cnt = e.y;
+    if ((uint32_t)lane < cnt) {
+      const uint32_t j = base + lane;
+      sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+      sh[lane] = pc.h[j];
+      P::stage(a, j, tile, lane);
+    }
+    const uint2 m = vl.mask[(size_t)(v0 + v) * kWave + lane];
+    __syncthreads();
+    if (active) {
+      uint32_t mb = m.x | m.y;
+      while (mb) {
+        const uint32_t b = (uint32_t)__builtin_ctz(mb);
+        mb &= mb - 1;
+        const uint32_t q = sub + b * kSub, jq = base + q;
+        const double dx[3] = {xi[0] - sx[q], xi[1] - sy[q], xi[2] - sz[q]};
+        const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+        const bool fo = (((m.x >> b) & 1u) != 0) & P::own_ok(t, tile, q, jq, i) &
+                        !pc.cut.outside(amax, P::hker_own(hi, sh[q]));
+        const bool fn = (((m.y >> b) & 1u) != 0) & P::nbr_ok(t, tile, q, jq, i) &
+                        !pc.cut.outside(amax, P::hker_nbr(hi, sh[q]));
+        if (fo | fn) P::pair(a, t, tile, q, dx, amax, hi, sh[q], fo, fn);
       }
     }
+    __syncthreads();
+  }
   P::combine(t);
   if (active && sub == 0) P::finish(a, t, i);
 }
@@ -703,6 +783,10 @@ struct Adjoint {
   uint32_t *cfirst = nullptr, *ccount = nullptr, *nchunks = nullptr;
   uint32_t *tcell = nullptr;
   size_t n_cap = 0, rows_cap = 0, chunk_cap = 0, tcell_cap = 0;
+  // visit lists of the step being reversed
+  uint32_t *vcount = nullptr, *voff = nullptr, *vnum = nullptr;
+  uint2 *vent = nullptr, *vmask = nullptr;
+  size_t vchunk_cap = 0, vis_cap = 0;
 };
 
 static void dev_fields(psph_ctx *c, double **d, uint32_t **u, int32_t **i) {
@@ -750,7 +834,7 @@ static void adjoint_free(psph_ctx *c) {
   for (auto p : A->l) if (p) hipFree(p);
   for (auto p : A->aux) if (p) hipFree(p);
   void *ptrs[] = {A->part, A->flags, A->tpos, A->rows, A->cchunks, A->coff, A->cfirst, A->ccount,
-                  A->nchunks, A->tcell};
+                  A->nchunks, A->tcell, A->vcount, A->voff, A->vnum, A->vent, A->vmask};
   for (auto p : ptrs) if (p) hipFree(p);
   delete A;
   c->adj = nullptr;
@@ -958,9 +1042,42 @@ static int adj_rows(psph_ctx *c, Adjoint &A, const StepRec &R) {
   return 0;
 }
 
-template <class P, int ROLES = 3>
-static void launch_adj(psph_ctx *c, const AdjCommon &pc, const StepRec &R, const typename P::Args &a) {
-  hipLaunchKernelGGL((k_adj_pairs<P, ROLES>), chunk_grid(R.n, R.grid.ncells, kChunk), kWave, 0, c->stream, pc, a);
+// visit lists of the step's row chunks (count, scan, masks)
+static int adj_visits(psph_ctx *c, Adjoint &A, const StepRec &R, const AdjCommon &pc, AdjVisits &vl) {
+  const unsigned G = chunk_grid(R.n, R.grid.ncells, kChunk);
+  if ((size_t)G + 1 > A.vchunk_cap) {
+    uint32_t **arrs[] = {&A.vcount, &A.voff, &A.vnum};
+    for (auto pp : arrs) { if (*pp) hipFree(*pp); *pp = nullptr; }
+    for (auto pp : arrs) PSPH_TRY(dmalloc((void **)pp, ((size_t)G + 1) * 4));
+    A.vchunk_cap = (size_t)G + 1;
+  }
+  hipLaunchKernelGGL(k_adj_visit_count, G, kWave, 0, c->stream, pc, A.vcount);
+  hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, c->stream, (int *)A.vcount + G, 0, 1);
+  HIP_TRY(hipGetLastError());
+  PSPH_TRY(excl_scan(c, A.vcount, A.voff, (size_t)G + 1));
+  uint32_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, A.voff + G, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if ((size_t)total > A.vis_cap) {
+    if (A.vmask) hipFree(A.vmask);
+    if (A.vent) hipFree(A.vent);
+    A.vmask = A.vent = nullptr;
+    const size_t cap = (size_t)total + total / 8 + 1024;
+    PSPH_TRY(dmalloc((void **)&A.vmask, cap * kWave * sizeof(uint2)));
+    PSPH_TRY(dmalloc((void **)&A.vent, cap * sizeof(uint2)));
+    A.vis_cap = cap;
+  }
+  hipLaunchKernelGGL(k_adj_visit_masks, G, kWave, 0, c->stream, pc, c->cfg.cutoff * (1.0 + 1e-12), A.voff, A.vnum,
+                     A.vent, A.vmask);
+  HIP_TRY(hipGetLastError());
+  vl = AdjVisits{A.voff, A.vnum, A.vent, A.vmask};
+  return 0;
+}
+
+template <class P>
+static void launch_adj(psph_ctx *c, const AdjCommon &pc, const AdjVisits &vl, const StepRec &R,
+                       const typename P::Args &a) {
+  hipLaunchKernelGGL((k_adj_pairs<P>), chunk_grid(R.n, R.grid.ncells, kChunk), kWave, 0, c->stream, pc, vl, a);
 }
 
 struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };
@@ -1027,13 +1144,15 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
   pc.list_all = k.list_all ? 1 : 0;
   pc.list_factor = k.list_factor; pc.cutoff2 = k.cutoff2;
   pc.cut = cut_test(k.cutoff);
+  AdjVisits vl;
+  PSPH_TRY(adj_visits(c, A, R, pc, vl));
   // 3. velocity filter
   if (k.lans) {
     hipLaunchKernelGGL(k_adj_vsm_pre, blocks(n), 256, 0, st, n, R.bc, R.d[RF_WNV], R.d[RF_SV2X], R.d[RF_SV2Y],
                        R.d[RF_SV2Z], l, A.aux[0], A.aux[1], A.aux[2], A.aux[3]);
     VsmAdj::Args a{R.d[RF_M], R.d[RF_RHO2], R.d[RF_H], R.d[RF_V1X], R.d[RF_V1Y], R.d[RF_V1Z], A.aux[0], A.aux[1],
                    A.aux[2], A.aux[3], R.bc, l};
-    launch_adj<VsmAdj>(c, pc, R, a);
+    launch_adj<VsmAdj>(c, pc, vl, R, a);
   }
   // 4. density + EOS
   {
@@ -1045,7 +1164,7 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
     DensAdj::Args a{R.d[RF_M], R.d[RF_RHO], R.d[RF_E1], R.bc, R.phase, A.aux[0], A.aux[1], A.aux[2], A.aux[3],
                     k.e_smooth_ratio, k.pghost_contrib, k.energy_smooth,
                     k.e_smooth_ratio * k.e_smooth_ratio == 16.0 ? 1 : 0, cut_test(k.cutoff), l};
-    launch_adj<DensAdj>(c, pc, R, a);
+    launch_adj<DensAdj>(c, pc, vl, R, a);
   }
   // 5. momentum / energy
   {
@@ -1059,7 +1178,7 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
     a.a0 = A.aux[0]; a.a1 = A.aux[1]; a.a2 = A.aux[2]; a.ae = A.aux[3];
     a.k = RhsAdjConst{k.alpha, k.beta, k.ata, k.eps_half, k.prandtl, k.lans};
     a.l = l;
-    launch_adj<RhsAdj>(c, pc, R, a);
+    launch_adj<RhsAdj>(c, pc, vl, R, a);
   }
   hipLaunchKernelGGL(k_lam_scatter, blocks(n), 256, 0, st, n, R.pid, l, Lid);
   HIP_TRY(hipGetLastError());
