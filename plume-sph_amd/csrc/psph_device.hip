@@ -505,10 +505,35 @@ __global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *
 }
 
 // the hit masks; every lane writes its mask of a visited tile, lane 0 the entry
+template <int D, bool FAST>
+__device__ inline uint32_t visit_mask(const PairCommon &pc, const double *xi, uint32_t sub, uint32_t cnt,
+                                      const double *sx, const double *sy, const double *sz, const double *shh,
+                                      const uint32_t *srefb, uint32_t rb_i, double hi, double hh_i, double lf_i,
+                                      double supp_i, double cutx) {
+  uint32_t mask = 0;
+  const uint32_t nb = (cnt + kSub - 1) / kSub;
+#pragma unroll 4
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint32_t q = sub + b * kSub;
+    double amax = fabs(xi[0] - sx[q]);
+    if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
+    bool in = amax <= cutx * fmax(hh_i + shh[q], hi);
+    if (!FAST) {
+      in &= (pc.list_all != 0) | (srefb[q] == rb_i) | (amax <= lf_i);
+      in &= amax <= supp_i;
+    }
+    mask |= (uint32_t)in << b;
+  }
+  return mask;
+}
+
+// FAST: for every target of the chunk the widened box lies inside both its
+// list box (4.25 h at list time) and its in_support box (CUTOFF2 h), so those
+// two tests cannot fail for a candidate inside the widened box.
 template <int D>
 __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const double *h, double cutoff2, double cutx,
-                                                       const uint32_t *voff, uint32_t *vnum, uint2 *vent,
-                                                       uint32_t *vmask) {
+                                                       double h_max, const uint32_t *voff, uint32_t *vnum,
+                                                       uint2 *vent, uint32_t *vmask) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], shh[kWave];
   __shared__ uint32_t srefb[kWave];
   const uint32_t nch = *pc.nchunks;
@@ -526,6 +551,8 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
   const uint32_t rb_i = pc.refb[i];
   const double hi = h[i], hh_i = 0.5 * hi;
   const double lf_i = pc.list_factor * pc.h_list[i], supp_i = cutoff2 * hi;
+  const double widest = cutx * fmax(hh_i + 0.5 * h_max, hi);
+  const bool fast = __all(!active || (widest <= supp_i && (pc.list_all != 0 || widest <= lf_i)));
   const ChunkBox box = chunk_box(pc.tr, xi);
   const GridDesc &g = pc.tr.g;
   const uint32_t w0 = voff[chunk];
@@ -551,19 +578,10 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
           __syncthreads();
           const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
           uint32_t mask = 0;
-          if (active) {
-            const uint32_t nb = (cnt + kSub - 1) / kSub;
-#pragma unroll 4
-            for (uint32_t b = 0; b < nb; b++) {
-              const uint32_t q = sub + b * kSub;
-              double amax = fabs(xi[0] - sx[q]);
-              if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
-              bool in = amax <= cutx * fmax(hh_i + shh[q], hi);
-              in &= (pc.list_all != 0) | (srefb[q] == rb_i) | (amax <= lf_i);
-              in &= amax <= supp_i;
-              mask |= (uint32_t)in << b;
-            }
-          }
+          if (active)
+            mask = fast ? visit_mask<D, true>(pc, xi, sub, cnt, sx, sy, sz, shh, srefb, rb_i, hi, hh_i, lf_i, supp_i, cutx)
+                        : visit_mask<D, false>(pc, xi, sub, cnt, sx, sy, sz, shh, srefb, rb_i, hi, hh_i, lf_i, supp_i,
+                                               cutx);
           if (__ballot(mask != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
             vmask[(size_t)w * kWave + lane] = mask;
@@ -1948,11 +1966,11 @@ static int ensure_visits(psph_ctx *c) {
   // a superset of the three passes' kernel boxes with a relative margin
   const double cutx = c->cfg.cutoff * (1.0 + 1e-12);
   if (c->cfg.dim == 3)
-    hipLaunchKernelGGL(k_visit_masks<3>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->voff,
-                       c->vnum, c->vent, c->vmask);
+    hipLaunchKernelGGL(k_visit_masks<3>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->h_max,
+                       c->voff, c->vnum, c->vent, c->vmask);
   else
-    hipLaunchKernelGGL(k_visit_masks<1>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->voff,
-                       c->vnum, c->vent, c->vmask);
+    hipLaunchKernelGGL(k_visit_masks<1>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->h_max,
+                       c->voff, c->vnum, c->vent, c->vmask);
   HIP_TRY(hipGetLastError());
   c->vis_valid = true;
   return 0;
