@@ -320,7 +320,7 @@ def main():
     bytes_per_launch = RHS_BYTES_PER_UPDATE * n_real
     achieved = bytes_per_launch / rhs_s / 1e9
     traffic = traffic_from_profiles("rhs_" + args.precision)
-    kname = "k_pairs<3, Bound<3, RhsPass<%s>>> (momentum/energy RHS)" % ("float" if args.precision == "fp32-pair" else "double")
+    kname = "k_pairs_v<3, Bound<3, RhsPass<%s>>> (momentum/energy RHS)" % ("float" if args.precision == "fp32-pair" else "double")
     roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

@@ -187,7 +187,7 @@ static_assert(kWave / kSub <= 32, "the adjoint's role masks hold one bit per can
 struct AdjVisits {
   const uint32_t *off, *num;
   const uint2 *ent;          // (first particle, particle count) of the visited tile
-  const uint2 *mask;         // per visit and lane: (own-role mask, neighbour-role mask)
+  const uint64_t *mask;      // per visit and lane: own-role mask | neighbour-role mask << 32
 };
 
 __global__ __launch_bounds__(kWave) void k_adj_visit_count(AdjCommon pc, uint32_t *vcount) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_count(AdjCommon pc, uint32_
 }
 
 __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double cutx, const uint32_t *voff,
-                                                           uint32_t *vnum, uint2 *vent, uint2 *vmask) {
+                                                           uint32_t *vnum, uint2 *vent, uint64_t *vmask) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], slf[kWave];
   __shared__ uint32_t srefb[kWave];
   __shared__ int sbc[kWave];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
           }
           if (__ballot((mo | mn) != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
-            vmask[(size_t)w * kWave + lane] = make_uint2(mo, mn);
+            __builtin_nontemporal_store((uint64_t)mo | (uint64_t)mn << 32, vmask + (size_t)w * kWave + lane);
             w++;
           }
           __syncthreads();
@@ -329,7 +329,8 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, AdjVisits vl,
       sh[lane] = pc.h[j];
       P::stage(a, j, tile, lane);
     }
-    const uint2 m = vl.mask[(size_t)(v0 + v) * kWave + lane];
+    const uint64_t m64 = __builtin_nontemporal_load(vl.mask + (size_t)(v0 + v) * kWave + lane);
+    const uint2 m = make_uint2((uint32_t)m64, (uint32_t)(m64 >> 32));
     __syncthreads();
     if (active) {
       uint32_t mb = m.x | m.y;
@@ -785,7 +786,8 @@ struct Adjoint {
   size_t n_cap = 0, rows_cap = 0, chunk_cap = 0, tcell_cap = 0;
   // visit lists of the step being reversed
   uint32_t *vcount = nullptr, *voff = nullptr, *vnum = nullptr;
-  uint2 *vent = nullptr, *vmask = nullptr;
+  uint2 *vent = nullptr;
+  uint64_t *vmask = nullptr;
   size_t vchunk_cap = 0, vis_cap = 0;
 };
 
@@ -1061,9 +1063,10 @@ static int adj_visits(psph_ctx *c, Adjoint &A, const StepRec &R, const AdjCommon
   if ((size_t)total > A.vis_cap) {
     if (A.vmask) hipFree(A.vmask);
     if (A.vent) hipFree(A.vent);
-    A.vmask = A.vent = nullptr;
+    A.vmask = nullptr;
+    A.vent = nullptr;
     const size_t cap = (size_t)total + total / 8 + 1024;
-    PSPH_TRY(dmalloc((void **)&A.vmask, cap * kWave * sizeof(uint2)));
+    PSPH_TRY(dmalloc((void **)&A.vmask, cap * kWave * sizeof(uint64_t)));
     PSPH_TRY(dmalloc((void **)&A.vent, cap * sizeof(uint2)));
     A.vis_cap = cap;
   }

@@ -584,7 +584,7 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
                                                cutx);
           if (__ballot(mask != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
-            vmask[(size_t)w * kWave + lane] = mask;
+            __builtin_nontemporal_store(mask, vmask + (size_t)w * kWave + lane);
             w++;
           }
           __syncthreads();
@@ -627,7 +627,8 @@ __global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, 
       if (pc.gflag) { const int b = pc.bc[j]; sgh[lane] = b == 1 || b == 2; }
       P::stage(a, j, tile, lane);
     }
-    uint32_t mask = vl.mask[(size_t)(v0 + v) * kWave + lane];
+    // streamed once per pass: keep it out of the caches that hold the tiles
+    uint32_t mask = __builtin_nontemporal_load(vl.mask + (size_t)(v0 + v) * kWave + lane);
     __syncthreads();
     if (active) {
       while (mask) {
