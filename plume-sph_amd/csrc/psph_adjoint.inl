@@ -14,11 +14,11 @@
 // intermediates (StepRec, ~256 B/particle/step), then sweeps them backwards.
 // The adjoint variables live in arrays indexed by persistent particle id, so
 // each step's cell-order permutation is a gather on entry and a scatter on
-// exit.  Pair terms are reversed by a gather over the same cell-row chunks as
-// the forward passes: a row particle collects both its own-role terms (it is
-// the particle whose sum holds the pair) and its neighbour-role terms (it is
-// the neighbour in a candidate's sum), so no atomics are needed and sums are
-// deterministic.  All adjoint arithmetic is FP64.
+// exit.  Pair terms are reversed by a gather over chunks of rows, built like
+// the forward pass's target chunks: a row particle collects both its own-role
+// terms (it is the particle whose sum holds the pair) and its neighbour-role
+// terms (it is the neighbour in a candidate's sum), so no atomics are needed
+// and sums are deterministic.  All adjoint arithmetic is FP64.
 
 namespace {
 
