@@ -505,27 +505,17 @@ __global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *
 }
 
 // the hit masks; every lane writes its mask of a visited tile, lane 0 the entry
-template <int D, bool FAST>
-__device__ inline uint32_t visit_mask(const PairCommon &pc, const double *xi, uint32_t sub, uint32_t cnt,
-                                      const double *sx, const double *sy, const double *sz, const double *shh,
-                                      const uint32_t *srefb, uint32_t rb_i, double hi, double hh_i, double lf_i,
-                                      double supp_i, double cutx) {
-  uint32_t mask = 0;
-  const uint32_t nb = (cnt + kSub - 1) / kSub;
-#pragma unroll 4
-  for (uint32_t b = 0; b < nb; b++) {
-    const uint32_t q = sub + b * kSub;
-    double amax = fabs(xi[0] - sx[q]);
-    if (D == 3) amax = fmax(amax, fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
-    bool in = amax <= cutx * fmax(hh_i + shh[q], hi);
-    if (!FAST) {
-      in &= (pc.list_all != 0) | (srefb[q] == rb_i) | (amax <= lf_i);
-      in &= amax <= supp_i;
-    }
-    mask |= (uint32_t)in << b;
-  }
-  return mask;
-}
+// The scan is transposed: lane l holds candidate c(l) = 2 (l mod 32) + l / 32
+// of the tile (so the ballot of a test over the 64 lanes is, in its low and
+// high halves, exactly the masks of a target's sub-lanes 0 and 1), and the
+// wave loops over the chunk's targets.  A target is skipped for a tile when
+// its widened box misses the tile's box (one vector test for all 32 targets,
+// then a scalar loop over the survivors), which drops about half of the
+// (target, tile) combinations of a chunk's grown box.
+struct VisitTgt {
+  double x, y, z, hh;        // position, h / 2
+  double h, lf, supp, wide;  // h, list box, in_support box, widest union box
+};
 
 // FAST: for every target of the chunk the widened box lies inside both its
 // list box (4.25 h at list time) and its in_support box (CUTOFF2 h), so those
@@ -534,8 +524,9 @@ template <int D>
 __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const double *h, double cutoff2, double cutx,
                                                        double h_max, const uint32_t *voff, uint32_t *vnum,
                                                        uint2 *vent, uint32_t *vmask) {
-  __shared__ double sx[kWave], sy[kWave], sz[kWave], shh[kWave];
-  __shared__ uint32_t srefb[kWave];
+  static_assert(kSub == 2, "the transposed scan maps the ballot halves onto two sub-lanes");
+  __shared__ VisitTgt tg[kChunk];
+  __shared__ uint32_t trb[kChunk];
   const uint32_t nch = *pc.nchunks;
   const uint32_t per = (nch + kXcds - 1) / kXcds;
   const uint32_t local = blockIdx.x / kXcds;
@@ -544,17 +535,33 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
   if (chunk >= nch) return;
   const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
   const int lane = threadIdx.x;
-  const uint32_t slot = (uint32_t)lane / kSub, sub = (uint32_t)lane % kSub;
-  const bool active = slot < cc;
-  const uint32_t i = pc.targets[cf + (active ? slot : 0)];
+  // lanes 0..cc-1 own one target each: its record in LDS, its float box in registers
+  const bool owner = (uint32_t)lane < cc;
+  const uint32_t i = pc.targets[cf + (owner ? (uint32_t)lane : 0u)];
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
-  const uint32_t rb_i = pc.refb[i];
   const double hi = h[i], hh_i = 0.5 * hi;
   const double lf_i = pc.list_factor * pc.h_list[i], supp_i = cutoff2 * hi;
   const double widest = cutx * fmax(hh_i + 0.5 * h_max, hi);
-  const bool fast = __all(!active || (widest <= supp_i && (pc.list_all != 0 || widest <= lf_i)));
-  const ChunkBox box = chunk_box(pc.tr, xi);
+  if (owner) {
+    tg[lane] = VisitTgt{xi[0], xi[1], xi[2], hh_i, hi, lf_i, supp_i, widest};
+    trb[lane] = pc.refb[i];
+  }
+  float blo[3], bhi[3];
+  for (int d = 0; d < 3; d++) {
+    blo[d] = D == 3 || d == 0 ? __double2float_rd(xi[d] - widest) : -INFINITY;
+    bhi[d] = D == 3 || d == 0 ? __double2float_ru(xi[d] + widest) : INFINITY;
+  }
+  const bool fast = __all(!owner || (widest <= supp_i && (pc.list_all != 0 || widest <= lf_i)));
+  // the chunk's grown box (same as the passes': every lane's target counts)
+  const uint32_t slot = (uint32_t)lane / kSub;
+  const uint32_t ib = pc.targets[cf + (slot < cc ? slot : 0u)];
+  const double xb[3] = {pc.x[ib], pc.y[ib], pc.z[ib]};
+  const ChunkBox box = chunk_box(pc.tr, xb);
+  __syncthreads();
   const GridDesc &g = pc.tr.g;
+  const uint32_t cq = 2u * ((uint32_t)lane & 31u) + ((uint32_t)lane >> 5);   // this lane's candidate
+  const uint32_t mine = (uint32_t)lane >> 1;                                 // this lane's target slot
+  const bool odd = (lane & 1) != 0;
   const uint32_t w0 = voff[chunk];
   uint32_t w = w0;
   for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
@@ -565,29 +572,42 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
         uint32_t tl = pc.tr.toff[cell];
         for (uint32_t base = lo; base < hi_; base += kWave, tl++) {
           if (!tile_meets(box, pc.tr.tbox, tl)) continue;
-          const uint32_t j = base + lane;
-          if (j < hi_) {
-            sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
-            shh[lane] = 0.5 * h[j];
-            srefb[lane] = pc.refb[j];
-          } else {
-            sx[lane] = sy[lane] = sz[lane] = kFar;
-            shh[lane] = 0;
-            srefb[lane] = ~0u;
-          }
-          __syncthreads();
           const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
+          // targets whose widened box meets the tile's box
+          const float4 tlo = pc.tr.tbox[2 * tl], thi = pc.tr.tbox[2 * tl + 1];
+          bool meets = owner && !(tlo.x > bhi[0] || thi.x < blo[0]);
+          if (D == 3) meets = meets && !(tlo.y > bhi[1] || thi.y < blo[1] || tlo.z > bhi[2] || thi.z < blo[2]);
+          uint64_t tm = __ballot(meets);
+          if (!tm) continue;
+          const bool valid = cq < cnt;
+          const uint32_t j = base + (valid ? cq : 0u);
+          const double cxp = valid ? pc.x[j] : kFar;
+          const double cyp = valid ? pc.y[j] : kFar;
+          const double czp = valid ? pc.z[j] : kFar;
+          const double chh = valid ? 0.5 * h[j] : 0.0;
+          const uint32_t crb = valid ? pc.refb[j] : ~0u;
           uint32_t mask = 0;
-          if (active)
-            mask = fast ? visit_mask<D, true>(pc, xi, sub, cnt, sx, sy, sz, shh, srefb, rb_i, hi, hh_i, lf_i, supp_i, cutx)
-                        : visit_mask<D, false>(pc, xi, sub, cnt, sx, sy, sz, shh, srefb, rb_i, hi, hh_i, lf_i, supp_i,
-                                               cutx);
+          while (tm) {
+            const uint32_t t = (uint32_t)__builtin_ctzll(tm);
+            tm &= tm - 1;
+            const VisitTgt T = tg[t];
+            double amax = fabs(T.x - cxp);
+            if (D == 3) amax = fmax(amax, fmax(fabs(T.y - cyp), fabs(T.z - czp)));
+            bool in = amax <= cutx * fmax(T.hh + chh, T.h);
+            if (!fast) {
+              in &= (pc.list_all != 0) | (crb == trb[t]) | (amax <= T.lf);
+              in &= amax <= T.supp;
+            }
+            const uint64_t bal = __ballot(in);
+            // lanes 2t and 2t+1 (target t's sub-lanes) take the two halves
+            const uint32_t half = odd ? (uint32_t)(bal >> 32) : (uint32_t)bal;
+            mask = mine == t ? half : mask;
+          }
           if (__ballot(mask != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
             __builtin_nontemporal_store(mask, vmask + (size_t)w * kWave + lane);
             w++;
           }
-          __syncthreads();
         }
       }
   if (lane == 0) vnum[chunk] = w - w0;
