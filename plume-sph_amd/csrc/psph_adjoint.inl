@@ -202,15 +202,7 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_count(AdjCommon pc, uint32_
   const uint32_t i = pc.rows[cf + (slot < cc ? slot : 0)];
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
   const ChunkBox box = chunk_box(pc.tr, xi);
-  const GridDesc &g = pc.tr.g;
-  uint32_t cnt = 0;
-  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
-    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
-      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
-        const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
-        const uint32_t ntile = pc.tr.toff[cell + 1] - pc.tr.toff[cell];
-        for (uint32_t t = 0; t < ntile; t++) cnt += tile_meets(box, pc.tr.tbox, pc.tr.toff[cell] + t) ? 1u : 0u;
-      }
+  const uint32_t cnt = count_meeting_tiles(pc.tr, box);
   if (threadIdx.x == 0) vcount[chunk] = cnt;
 }
 

@@ -479,7 +479,23 @@ struct VisitList {
 };
 
 // tiles that meet each chunk's grown box (an upper bound on its visits);
-// one wavefront per chunk, chunk = blockIdx.x, zero past the last chunk
+// one wavefront per chunk, chunk = blockIdx.x, zero past the last chunk; the
+// lanes take the box's cells in turn
+__device__ inline uint32_t count_meeting_tiles(const Traversal &tr, const ChunkBox &box) {
+  const GridDesc &g = tr.g;
+  const int bx = box.c1[0] - box.c0[0] + 1, by = box.c1[1] - box.c0[1] + 1, bz = box.c1[2] - box.c0[2] + 1;
+  const int nbox = bx * by * bz;
+  uint32_t cnt = 0;
+  for (int k = threadIdx.x; k < nbox; k += kWave) {
+    const int cx = box.c0[0] + k % bx, cy = box.c0[1] + (k / bx) % by, cz = box.c0[2] + k / (bx * by);
+    const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
+    const uint32_t t0 = tr.toff[cell], t1 = tr.toff[cell + 1];
+    for (uint32_t t = t0; t < t1; t++) cnt += tile_meets(box, tr.tbox, t) ? 1u : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, o);
+  return cnt;
+}
+
 __global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *vcount) {
   const uint32_t chunk = blockIdx.x;
   const uint32_t nch = *pc.nchunks;
@@ -492,19 +508,10 @@ __global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *
   const uint32_t i = pc.targets[cf + (slot < cc ? slot : 0)];
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
   const ChunkBox box = chunk_box(pc.tr, xi);
-  const GridDesc &g = pc.tr.g;
-  uint32_t cnt = 0;
-  for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
-    for (int cy = box.c0[1]; cy <= box.c1[1]; cy++)
-      for (int cx = box.c0[0]; cx <= box.c1[0]; cx++) {
-        const uint32_t cell = ((uint32_t)cz * g.ny + cy) * g.nx + cx;
-        const uint32_t ntile = pc.tr.toff[cell + 1] - pc.tr.toff[cell];
-        for (uint32_t t = 0; t < ntile; t++) cnt += tile_meets(box, pc.tr.tbox, pc.tr.toff[cell] + t) ? 1u : 0u;
-      }
+  const uint32_t cnt = count_meeting_tiles(pc.tr, box);
   if (threadIdx.x == 0) vcount[chunk] = cnt;
 }
 
-// the hit masks; every lane writes its mask of a visited tile, lane 0 the entry
 // The scan is transposed: lane l holds candidate c(l) = 2 (l mod 32) + l / 32
 // of the tile (so the ballot of a test over the 64 lanes is, in its low and
 // high halves, exactly the masks of a target's sub-lanes 0 and 1), and the
@@ -513,8 +520,8 @@ __global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *
 // then a scalar loop over the survivors), which drops about half of the
 // (target, tile) combinations of a chunk's grown box.
 struct VisitTgt {
-  double x, y, z, hh;        // position, h / 2
-  double h, lf, supp, wide;  // h, list box, in_support box, widest union box
+  double x, y, z, chh;       // position, cutx h / 2
+  double ch, lf, supp, wide; // cutx h, list box, in_support box, widest union box
 };
 
 // FAST: for every target of the chunk the widened box lies inside both its
@@ -543,7 +550,7 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
   const double lf_i = pc.list_factor * pc.h_list[i], supp_i = cutoff2 * hi;
   const double widest = cutx * fmax(hh_i + 0.5 * h_max, hi);
   if (owner) {
-    tg[lane] = VisitTgt{xi[0], xi[1], xi[2], hh_i, hi, lf_i, supp_i, widest};
+    tg[lane] = VisitTgt{xi[0], xi[1], xi[2], cutx * hh_i, cutx * hi, lf_i, supp_i, widest};
     trb[lane] = pc.refb[i];
   }
   float blo[3], bhi[3];
@@ -584,7 +591,7 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
           const double cxp = valid ? pc.x[j] : kFar;
           const double cyp = valid ? pc.y[j] : kFar;
           const double czp = valid ? pc.z[j] : kFar;
-          const double chh = valid ? 0.5 * h[j] : 0.0;
+          const double chh = valid ? cutx * (0.5 * h[j]) : 0.0;
           const uint32_t crb = valid ? pc.refb[j] : ~0u;
           uint32_t mask = 0;
           while (tm) {
@@ -593,7 +600,9 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
             const VisitTgt T = tg[t];
             double amax = fabs(T.x - cxp);
             if (D == 3) amax = fmax(amax, fmax(fabs(T.y - cyp), fabs(T.z - czp)));
-            bool in = amax <= cutx * fmax(T.hh + chh, T.h);
+            // amax <= cutx max(h_ij, h_t), with cutx h_ij ~ cutx hh_t + cutx hh_j
+            // (the 1e-12 margin of cutx covers the rounding of the split sum)
+            bool in = (amax <= T.chh + chh) | (amax <= T.ch);
             if (!fast) {
               in &= (pc.list_all != 0) | (crb == trb[t]) | (amax <= T.lf);
               in &= amax <= T.supp;
