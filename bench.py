@@ -248,7 +248,7 @@ def main():
                     help="form whose throughput is the reported value")
     ap.add_argument("--no-compare", action="store_true", help="skip the run of the other form")
     ap.add_argument("--cpu-sample", type=float, default=6.0e4, help="real particles in the CPU sample")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=15, help="steps of the CPU sample (~10 s on 16 cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--adjoint-particles", type=float, default=5.12e5)
     ap.add_argument("--adjoint-steps", type=int, default=200)

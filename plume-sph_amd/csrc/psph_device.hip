@@ -725,7 +725,11 @@ template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
 template <class R> __device__ inline R rrcp(R x);
 template <> __device__ inline double rrcp<double>(double x) { return 1.0 / x; }
 template <> __device__ inline float rrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-template <class R> __device__ inline R rabs(R x) { return x < R(0) ? -x : x; }
+// |x| as the source modifier of the consuming instruction (x < 0 ? -x : x
+// would cost a compare and a select)
+template <class R> __device__ inline R rabs(R x);
+template <> __device__ inline double rabs<double>(double x) { return __builtin_fabs(x); }
+template <> __device__ inline float rabs<float>(float x) { return __builtin_fabsf(x); }
 
 // exp(-e) in FP64 for 0 <= e < kETab / 16 from a table of exp(-k/16) and the
 // Taylor series of exp(-r), 0 <= r < 1/16, to r^8 (truncation < 4e-17).
