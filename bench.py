@@ -41,18 +41,92 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def weak_plume_config(n_real: float, lx=8000.0, ly=8000.0, lz=12000.0):
-    """Weak plume: 8 x 8 x 12 km, N real particles (spacing (V/N)^(1/3)), vent
-    r = 54 m, 135 m/s, 1053 K, erupted-material gas fraction 0.03, p = p(0)."""
+def plume_config(n_real: float, dims, vent):
+    """Plume box dims = (lx, ly, lz) m with N real particles (spacing
+    (V/N)^(1/3)); vent = (radius m, velocity m/s, T K, gas fraction), p = p(0)."""
+    lx, ly, lz = dims
     cfg = psph.default_config(3)
     cfg.h_air = (lx * ly * lz / n_real) ** (1.0 / 3.0)
     for k, (lo, hi) in enumerate([(-lx / 2, lx / 2), (-ly / 2, ly / 2), (0.0, lz)]):
         cfg.dom_lo[k] = lo
         cfg.dom_hi[k] = hi
     cfg.vent_z = 0.0
-    cfg.vent_r, cfg.vent_v, cfg.vent_T, cfg.ng0, cfg.vent_p = 54.0, 135.0, 1053.0, 0.03, 101325.0
+    cfg.vent_r, cfg.vent_v, cfg.vent_T, cfg.ng0 = vent
+    cfg.vent_p = 101325.0
     psph.finalize_config(cfg)
     return cfg
+
+
+def weak_plume_config(n_real: float, lx=8000.0, ly=8000.0, lz=12000.0):
+    """Weak plume: 8 x 8 x 12 km, vent r = 54 m, 135 m/s, 1053 K, erupted-material
+    gas fraction 0.03."""
+    return plume_config(n_real, (lx, ly, lz), (54.0, 135.0, 1053.0, 0.03))
+
+
+def strong_plume_config(n_real: float):
+    """Strong plume: 40 x 40 x 30 km, vent r = 700 m, 150 m/s, 1273 K, erupted-
+    material gas fraction 0.05."""
+    return plume_config(n_real, (40000.0, 40000.0, 30000.0), (700.0, 150.0, 1273.0, 0.05))
+
+
+def strong_leg(args):
+    """Strong plume at scale: FP32-pair forward for 100 steps, then the adjoint
+    of the plume height over the last 50 with the checkpoint interval chosen
+    so that one segment of step records fills at most 30 % of the free HBM."""
+    sys.path.insert(0, os.path.join(ROOT, "plume-sph_amd"))
+    import torch
+    cfg = strong_plume_config(args.strong_particles)
+    cfg.precision = psph.PSPH_FP32_PAIR
+    met = psph.meteo_standard()
+    t0 = time.perf_counter()
+    P = psph.gen_plume(cfg, met, 20261206, 0.01)
+    n0 = len(P)
+    log(f"[bench] strong: {n0} particles, spacing {cfg.h_air:.1f} m, generated in {time.perf_counter() - t0:.1f}s")
+    S = psph.Solver(cfg, int(n0 * 1.2) + 200000)
+    S.load(P)
+    del P
+    first = args.strong_steps - args.strong_adjoint
+    sync()
+    t0 = time.perf_counter()
+    S.run(first)
+    sync()
+    w_first = time.perf_counter() - t0
+    t_sim, step = S.time()
+    Q = S.read()
+    _, n_real_mid = S.count()
+    S.load(Q, t_sim, step)
+    sync()
+    t0 = time.perf_counter()
+    S.run(args.strong_adjoint)
+    sync()
+    w_last = time.perf_counter() - t0
+    n_total, n_real = S.count()
+    log(f"[bench] strong: steps 1-{first} in {w_first:.1f}s, steps {first + 1}-{args.strong_steps} in "
+        f"{w_last:.1f}s ({1e3 * w_last / args.strong_adjoint:.1f} ms/step, {n_real} real of {n_total})")
+    free, _ = torch.cuda.mem_get_info()
+    # a step record holds 30 doubles and 6 words per particle of capacity; the
+    # adjoint's own arrays, visit lists and checkpoints take most of the rest
+    per_step = 264.0 * S.capacity
+    K = int(max(1, min(args.strong_adjoint, 0.3 * free / per_step)))
+    log(f"[bench] strong adjoint: {free / 1e9:.0f} GB free, {per_step / 1e9:.1f} GB per step record, "
+        f"checkpoint every {K}")
+    S.load(Q, t_sim, step)
+    del Q
+    r = S.adjoint_run(args.strong_adjoint, K, grads=False)
+    S.close()
+    log(f"[bench] strong adjoint: {args.strong_adjoint} steps, checkpoint every {K}: recorded forward "
+        f"{r['forward_ms']:.0f} ms, backward {r['backward_ms']:.0f} ms")
+    return {"workload": f"strong plume, 40x40x30 km, {args.strong_particles / 1e6:g}M real particles, FP32-pair "
+                        f"forward {args.strong_steps} steps, adjoint over the last {args.strong_adjoint}",
+            "particles_total": n_total, "particles_real": n_real, "spacing_m": cfg.h_air,
+            "forward_ms_per_step_last": 1e3 * w_last / args.strong_adjoint,
+            "forward_ms_per_step_first": 1e3 * w_first / first,
+            "particle_updates_per_s_last": 0.5 * (n_real_mid + n_real) * args.strong_adjoint / w_last,
+            "adjoint_checkpoint_every": K, "hbm_free_gb_at_adjoint": free / 1e9,
+            "H": r["H"], "dH_dvent_v": r["dH_dvent_v"], "dH_dvent_T": r["dH_dvent_T"],
+            "dH_dmass_frac": r["dH_dmass_frac"], "adjoint_recorded_forward_ms": r["forward_ms"],
+            "adjoint_backward_ms": r["backward_ms"],
+            "backward_over_forward": r["backward_ms"] / (1e3 * w_last)}
 
 
 def sync():
@@ -257,6 +331,11 @@ def main():
     ap.add_argument("--adjoint-short", type=int, default=20, help="horizon of the short FD check")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
+    ap.add_argument("--strong", action="store_true", help="also run the 16M strong-plume leg (minutes)")
+    ap.add_argument("--strong-only", action="store_true", help="run only the strong-plume leg")
+    ap.add_argument("--strong-particles", type=float, default=16.0e6)
+    ap.add_argument("--strong-steps", type=int, default=100)
+    ap.add_argument("--strong-adjoint", type=int, default=50, help="adjoint over the last N steps")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB: run only a plain device copy of this size (PMC calibration)")
     args = ap.parse_args()
@@ -279,6 +358,9 @@ def main():
 
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(args)}))
+        return
+    if args.strong_only:
+        print(json.dumps({"strong": strong_leg(args)}))
         return
 
     base = weak_plume_config(args.particles)
@@ -339,6 +421,14 @@ def main():
             log(f"[bench] adjoint leg failed: {e}")
             adjoint = {"error": str(e)}
 
+    strong = None
+    if args.strong:
+        try:
+            strong = strong_leg(args)
+        except Exception as e:   # reported as missing, the headline stands
+            log(f"[bench] strong leg failed: {e}")
+            strong = {"error": str(e)}
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -361,6 +451,7 @@ def main():
         "roofline": roofline, "roofline_valu": roofline_valu, "cpu_baseline": cpu,
         "forms": compare,
         "adjoint": adjoint,
+        "strong": strong,
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
     }
     print(json.dumps(out))
