@@ -318,7 +318,7 @@ def main():
                     help="untimed steps; >= 50 so the smoothing length has adapted once")
     ap.add_argument("--particles", type=float, default=4.0e6, help="real particles")
     ap.add_argument("--seed", type=int, default=20261204)
-    ap.add_argument("--precision", choices=["fp64", "fp32-pair"], default="fp32-pair",
+    ap.add_argument("--precision", choices=["fp64", "fp32-pair", "fp32-pure"], default="fp32-pair",
                     help="form whose throughput is the reported value")
     ap.add_argument("--no-compare", action="store_true", help="skip the run of the other form")
     ap.add_argument("--cpu-sample", type=float, default=6.0e4, help="real particles in the CPU sample")
@@ -372,11 +372,13 @@ def main():
     log(f"[bench] generated {n_total0} particles ({n_real0} real, spacing {base.h_air:.2f} m) "
         f"in {time.perf_counter() - t0:.1f}s")
 
-    forms = [args.precision] + ([] if args.no_compare else [p for p in ("fp32-pair", "fp64") if p != args.precision])
+    forms = [args.precision] + ([] if args.no_compare else
+                                [p for p in ("fp32-pair", "fp64", "fp32-pure") if p != args.precision])
+    codes = {"fp32-pair": psph.PSPH_FP32_PAIR, "fp32-pure": psph.PSPH_FP32_PURE, "fp64": psph.PSPH_FP64}
     runs = {}
     for form in forms:
         cfg = base.copy()
-        cfg.precision = psph.PSPH_FP32_PAIR if form == "fp32-pair" else psph.PSPH_FP64
+        cfg.precision = codes[form]
         runs[form] = run_form(cfg, P, args, form)
     del P
     main_run = runs[args.precision]
@@ -385,15 +387,20 @@ def main():
     cfg = base
 
     compare = None
-    if len(runs) == 2:
-        A, B = runs["fp32-pair"]["final"], runs["fp64"]["final"]
-        compare = {"fp64_value": runs["fp64"]["value"], "fp32_pair_value": runs["fp32-pair"]["value"],
-                   "speedup_fp32_pair_over_fp64": runs["fp32-pair"]["value"] / runs["fp64"]["value"],
-                   "rel_l1_fp32_pair_vs_fp64": rel_l1_by_key(A, B),
-                   "note": "both forms from the same input and step count; the FP64 form is the "
-                           "parity-checked one (1e-10 vs the CPU oracle on small cases)"}
-        log(f"[bench] fp32-pair / fp64 = {compare['speedup_fp32_pair_over_fp64']:.3f}; rel L1 "
-            + " ".join(f"{k}={v:.1e}" for k, v in compare["rel_l1_fp32_pair_vs_fp64"].items()))
+    if len(runs) == 3:
+        B = runs["fp64"]["final"]
+        compare = {"fp64_value": runs["fp64"]["value"],
+                   "note": "all forms from the same input and step count; the FP64 form is the parity-checked "
+                           "one (1e-10 vs the CPU oracle on small cases). fp32-pair keeps the RHS pressure term "
+                           "and the density / filter weights in FP64; fp32-pure runs the RHS and density pair "
+                           "arithmetic all in FP32 (faster, velocities ~1e-4 off)"}
+        for form in ("fp32-pair", "fp32-pure"):
+            key = form.replace("-", "_")
+            compare[f"{key}_value"] = runs[form]["value"]
+            compare[f"speedup_{key}_over_fp64"] = runs[form]["value"] / runs["fp64"]["value"]
+            compare[f"rel_l1_{key}_vs_fp64"] = rel_l1_by_key(runs[form]["final"], B)
+            log(f"[bench] {form} / fp64 = {compare[f'speedup_{key}_over_fp64']:.3f}; rel L1 "
+                + " ".join(f"{k}={v:.1e}" for k, v in compare[f"rel_l1_{key}_vs_fp64"].items()))
     for r in runs.values():
         r.pop("final", None)
 
@@ -402,7 +409,8 @@ def main():
     bytes_per_launch = RHS_BYTES_PER_UPDATE * n_real
     achieved = bytes_per_launch / rhs_s / 1e9
     traffic = traffic_from_profiles("rhs_" + args.precision)
-    kname = "k_pairs_v<3, Bound<3, RhsPass<%s>>> (momentum/energy RHS)" % ("float" if args.precision == "fp32-pair" else "double")
+    kname = "k_pairs_v<3, Bound<3, RhsPass<%s>>> (momentum/energy RHS)" % {
+        "fp32-pair": "float", "fp32-pure": "float, false", "fp64": "double"}[args.precision]
     roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -441,10 +449,10 @@ def main():
         "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-pair+f64",
+        "dtype": {"fp64": "f64", "fp32-pair": "f32-pair+f64", "fp32-pure": "f32-pair"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"weak dusty-gas plume, 8x8x12 km, {args.particles / 1e6:g}M real particles, "
-                               f"{'FP64 form' if args.precision == 'fp64' else 'FP32-pair form'}",
+                               f"{ {'fp64': 'FP64 form', 'fp32-pair': 'FP32-pair form', 'fp32-pure': 'FP32-pure form'}[args.precision]}",
                    "particles_real": n_real, "particles_total": n_total_end,
                    "spacing_m": cfg.h_air, "vent": "r=54 m, 135 m/s, 1053 K, gas fraction 0.03",
                    "kernel": "gaussian-box", "steps_from": step - args.steps, "sim_time_s": t_sim},

@@ -773,10 +773,10 @@ struct RhsArgs {
 struct RhsMixTile { double m64[kWave], pv64[kWave], etab[kETab]; };
 struct NoTile {};
 
-template <class R>
+template <class R, bool MIXED = std::is_same<R, float>::value>
 struct RhsPass {
   using Args = RhsArgs;
-  static constexpr bool kMixed = std::is_same<R, float>::value;
+  static constexpr bool kMixed = MIXED;
   struct Tile : std::conditional<kMixed, RhsMixTile, NoTile>::type {
     double hh[kWave];       // h / 2
     R m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
@@ -991,10 +991,10 @@ struct DensMixTile { double m64[kWave], irho64[kWave], e64[kWave], etab[kETab]; 
 // The FP32-pair form evaluates the density and energy-smoothing weights in
 // FP64 with the exp table: the pressure gradient of the next step amplifies
 // relative errors of rho and e by its cancellation.
-template <class R>
+template <class R, bool MIXED = std::is_same<R, float>::value>
 struct DensPass {
   using Args = DensArgs;
-  static constexpr bool kMixed = std::is_same<R, float>::value;
+  static constexpr bool kMixed = MIXED;
   struct Tile : std::conditional<kMixed, DensMixTile, NoTile>::type {
     double hh[kWave];       // h / 2
     R m[kWave], rho[kWave], e[kWave];
@@ -1589,7 +1589,8 @@ static int ensure_cub(psph_ctx *c, size_t bytes) {
 extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **out) {
   if (!cfg || !out) return fail(PSPH_EINVAL, "null argument");
   if (cfg->dim != 1 && cfg->dim != 3) return fail(PSPH_EINVAL, "dim must be 1 or 3");
-  if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_PAIR) return fail(PSPH_EINVAL, "unknown precision form %d", cfg->precision);
+  if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_PAIR && cfg->precision != PSPH_FP32_PURE)
+    return fail(PSPH_EINVAL, "unknown precision form %d", cfg->precision);
   if (cfg->kernel != PSPH_KERNEL_GAUSS_BOX) return fail(PSPH_ENOTSUP, "only the Gaussian box kernel is built");
   if (!(cfg->h_air > 0)) return fail(PSPH_EINVAL, "h_air must be > 0");
   {
@@ -1960,13 +1961,6 @@ static void launch_pairs(psph_ctx *c, const PairCommon &pc, const typename P::Ar
     hipLaunchKernelGGL((k_pairs<1, Bound<1, P>>), nb, kWave, 0, c->stream, pc, a);
 }
 
-// FP64 form or FP32-pair form of a pass template
-template <template <class> class P>
-static void launch_form(psph_ctx *c, const PairCommon &pc, const typename P<double>::Args &a) {
-  if (c->cfg.precision == PSPH_FP32_PAIR) launch_pairs<P<float>>(c, pc, a);
-  else launch_pairs<P<double>>(c, pc, a);
-}
-
 // Visit lists of the current step (positions, h and targets fixed since the
 // last psph_neighbors / adaptation): count the tiles each chunk meets, scan,
 // then write the tiles with hits and their lane masks.
@@ -2010,20 +2004,27 @@ static int ensure_visits(psph_ctx *c) {
   return 0;
 }
 
-// the RHS, density and filter passes run over the visit lists
-template <template <class> class P>
-static int launch_visits(psph_ctx *c, const typename P<double>::Args &a) {
+// the RHS, density and filter passes run over the visit lists; PM, PP, PD:
+// the pass in the FP32-pair, FP32-pure and FP64 forms
+template <int D, class P>
+static void launch_visit_pass(psph_ctx *c, const PairCommon &pc, const VisitList &vl, const typename P::Args &a) {
+  const unsigned nb = chunk_grid(c->n, c->grid.ncells, kChunk);
+  hipLaunchKernelGGL((k_pairs_v<D, Bound<D, P>>), nb, kWave, 0, c->stream, pc, vl, a);
+}
+template <class PM, class PP, class PD>
+static int launch_visits(psph_ctx *c, const typename PD::Args &a) {
   PSPH_TRY(ensure_visits(c));
   const PairCommon pc = make_common(c, c->reach_main);
   const VisitList vl{c->voff, c->vnum, c->vent, c->vmask};
-  const unsigned nb = chunk_grid(c->n, c->grid.ncells, kChunk);
-  const bool f32 = c->cfg.precision == PSPH_FP32_PAIR;
+  const int form = c->cfg.precision;
   if (c->cfg.dim == 3) {
-    if (f32) hipLaunchKernelGGL((k_pairs_v<3, Bound<3, P<float>>>), nb, kWave, 0, c->stream, pc, vl, a);
-    else hipLaunchKernelGGL((k_pairs_v<3, Bound<3, P<double>>>), nb, kWave, 0, c->stream, pc, vl, a);
+    if (form == PSPH_FP32_PAIR) launch_visit_pass<3, PM>(c, pc, vl, a);
+    else if (form == PSPH_FP32_PURE) launch_visit_pass<3, PP>(c, pc, vl, a);
+    else launch_visit_pass<3, PD>(c, pc, vl, a);
   } else {
-    if (f32) hipLaunchKernelGGL((k_pairs_v<1, Bound<1, P<float>>>), nb, kWave, 0, c->stream, pc, vl, a);
-    else hipLaunchKernelGGL((k_pairs_v<1, Bound<1, P<double>>>), nb, kWave, 0, c->stream, pc, vl, a);
+    if (form == PSPH_FP32_PAIR) launch_visit_pass<1, PM>(c, pc, vl, a);
+    else if (form == PSPH_FP32_PURE) launch_visit_pass<1, PP>(c, pc, vl, a);
+    else launch_visit_pass<1, PD>(c, pc, vl, a);
   }
   return 0;
 }
@@ -2113,7 +2114,7 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 3);
-    if (n) PSPH_TRY(launch_visits<RhsPass>(c, a));
+    if (n) PSPH_TRY((launch_visits<RhsPass<float>, RhsPass<float, false>, RhsPass<double>>(c, a)));
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
@@ -2144,7 +2145,7 @@ extern "C" int psph_density(psph_ctx *c) {
   if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 4);
-    if (n) PSPH_TRY(launch_visits<DensPass>(c, a));
+    if (n) PSPH_TRY((launch_visits<DensPass<float>, DensPass<float, false>, DensPass<double>>(c, a)));
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.rho, c->dsp[1]);
@@ -2165,7 +2166,7 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   if (c->n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 5);
-    if (c->n) PSPH_TRY(launch_visits<VsmPass>(c, a));
+    if (c->n) PSPH_TRY((launch_visits<VsmPass<float>, VsmPass<float>, VsmPass<double>>(c, a)));
   }
   HIP_TRY(hipGetLastError());
   return check_err(c, "vsmooth");

@@ -22,7 +22,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpsph.so")
 
-PSPH_FP64, PSPH_FP32_PAIR = 0, 1
+PSPH_FP64, PSPH_FP32_PAIR, PSPH_FP32_PURE = 0, 1, 2
 PSPH_KERNEL_GAUSS_BOX, PSPH_KERNEL_CUBIC_M4 = 0, 1
 BC_ERUPT_GHOST, BC_PRESSURE_GHOST, BC_WALL_GHOST, BC_NOSAVE_REAL, BC_REAL = 0, 1, 2, 99, 100
 
