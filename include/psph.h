@@ -79,7 +79,10 @@ typedef struct psph_config {
   int32_t max_steps;        /* deck line 2                                     */
   int32_t num_erupt_particles; /* parameters.h:983                            */
   int32_t p_table_size;     /* P_TABLE_SIZE: slots of the particle hash table  */
-  int32_t reserved_i[2];
+  int32_t wall_bc;          /* 1: refresh the wall ghosts next to the ground
+                               from their mirror images after every step
+                               (apply_bcond, main.cc:468); 0: keep their state */
+  int32_t reserved_i[1];
 
   double cutoff;            /* CUTOFF = 3: kernel box half-width in h         */
   double cutoff2;           /* CUTOFF2: pair pre-filter box in h_i (8 / 12)   */
@@ -229,6 +232,13 @@ int psph_vsmooth(psph_ctx *ctx);
 int psph_update_pos(psph_ctx *ctx, double dt);
 /* add_new_erupt (setup_erupt.cc:389-670); *added receives the count. */
 int psph_inject(psph_ctx *ctx, double dt, int64_t *added);
+/* apply_bcond (bcond.cc:24-249): refresh the wall ghosts of the bucket layer
+ * that straddles the ground (z = dom_lo[2]) from their mirror images; runs a
+ * neighbour search first when none is current.  psph_step calls it at the end
+ * of every step when cfg.wall_bc is set (as main.cc:468 does); call it once
+ * after psph_load for the refresh the reference makes before its loop
+ * (main.cc:196). */
+int psph_apply_bcond(psph_ctx *ctx);
 /* One whole step in the order of main.cc:217-486 (3D) / 676-751 (1D). */
 int psph_step(psph_ctx *ctx, psph_step_out *out);
 int psph_run(psph_ctx *ctx, int64_t nsteps, psph_step_out *last);

@@ -85,6 +85,7 @@ def lib():
         "orc_remove_escaped": (C.c_int, [CFG, S]),
         "orc_inject": (C.c_int, [CFG, S, C.c_double]),
         "orc_step": (C.c_int, [CFG, S]),
+        "orc_apply_bcond": (C.c_int, [CFG, S]),
         "orc_iferupt": (C.c_int, [CFG, S]),
         "orc_conservation": (None, [S, C.POINTER(C.c_double)]),
         "orc_diag_height": (C.c_double, [S]),
@@ -182,6 +183,9 @@ class State:
 
     def step(self):
         _check(lib().orc_step(C.byref(self.cfg), C.byref(self.st)))
+
+    def apply_bcond(self):
+        _check(lib().orc_apply_bcond(C.byref(self.cfg), C.byref(self.st)))
 
     def conservation(self):
         out = (C.c_double * 7)()

@@ -1174,6 +1174,85 @@ double orc_diag_height(const orc_state *st) {
 /* ===================================================================== */
 /* One step in the reference order (main.cc:217-486 3D, 676-751 1D)        */
 /* ===================================================================== */
+/* ===================================================================== */
+/* apply_bcond (bcond.cc:24-249) with the images of search_bnd_images     */
+/* (bnd_image.cc:33-254) for a flat ground at z = dom_lo[2]: every wall   */
+/* ghost of a bucket that straddles the ground (bucket type MIXED) takes  */
+/* the state interpolated at its mirror point refc = (x, y, 2 z_g - z)    */
+/* from the real and pressure-ghost particles j of the image's bucket and */
+/* its 26 neighbours: Shepard sums of m_j W(s, h_j) (s = (refc - x_j)/h_j,*/
+/* in_support box CUTOFF2 h_j) over rho, v / rho, e / rho, normalised by  */
+/* sum m_j W / rho_j; the mass fraction sums only the neighbour buckets   */
+/* (bcond.cc:140-146, BC_FOR_KX 1); the velocity is reflected in the      */
+/* ground plane (sph_lib.cc:75-83) and the EOS re-evaluated.              */
+/* ===================================================================== */
+static int orc_contr_image(int bc) { return bc == 100 || bc == 99 || bc == 1; }   /* particle.h:562-565 */
+
+int orc_apply_bcond(const psph_config *cfg, orc_state *st) {
+  if (cfg->dim != 3) return 0;
+  const double zg = cfg->dom_lo[2];
+  const double fz = (zg - cfg->ext_lo[2]) / cfg->bucket_edge;
+  if (fz == floor(fz)) return 0;                  /* ground on a bucket face: no MIXED bucket */
+  const int bzg = (int)floor(fz);
+  const size_t n = st->n;
+  /* the new ghost states are computed from the old ones of the contributors only
+     (bc 100, 99, 1), and only wall ghosts (bc 2) are written: ghosts are independent */
+#pragma omp parallel for schedule(dynamic, 64)
+  for (size_t g = 0; g < n; g++) {
+    if (st->bc[g] != 2) continue;
+    int bg[3];
+    if (orc_bucket_of(cfg, st, g, bg) || bg[2] != bzg) continue;
+    double coord[3], refc[3], normal[3], d = 0;
+    for (int k = 0; k < 3; k++) coord[k] = st->x[k][g];
+    refc[0] = coord[0]; refc[1] = coord[1]; refc[2] = 2 * zg - coord[2];
+    for (int k = 0; k < 3; k++) { normal[k] = refc[k] - coord[k]; d += normal[k] * normal[k]; }
+    if (!(d > 0)) { normal[0] = normal[1] = 0; normal[2] = 1; d = 1; }
+    for (int k = 0; k < 3; k++) normal[k] /= sqrt(d);
+    /* bucket of the image */
+    int bi[3], nb[3], ok = 1;
+    for (int k = 0; k < 3; k++) {
+      nb[k] = (int)llround((cfg->ext_hi[k] - cfg->ext_lo[k]) / cfg->bucket_edge);
+      double f = floor((refc[k] - cfg->ext_lo[k]) / cfg->bucket_edge);
+      if (!(f >= 0 && f < nb[k])) ok = 0; else bi[k] = (int)f;
+    }
+    if (!ok) continue;
+    double uvec[5] = {0, 0, 0, 0, 0}, wnorm = 0, ks = 0;
+    for (size_t j = 0; j < n; j++) {
+      if (!orc_contr_image(st->bc[j])) continue;
+      int bj[3];
+      if (orc_bucket_of(cfg, st, j, bj)) continue;
+      if (abs(bj[0] - bi[0]) > 1 || abs(bj[1] - bi[1]) > 1 || abs(bj[2] - bi[2]) > 1) continue;
+      const double h = st->h[j], supp = cfg->cutoff2 * h;
+      double dx[3], sv[3];
+      int in = 1;
+      for (int k = 0; k < 3; k++) { dx[k] = refc[k] - st->x[k][j]; if (fabs(dx[k]) > supp) in = 0; }
+      if (!in) continue;
+      for (int k = 0; k < 3; k++) sv[k] = dx[k] / h;
+      const double w = orc_weight(sv, h, 3), mj = st->m[j], rj = st->rho[j];
+      uvec[0] += mj * w;
+      uvec[1] += mj * w * st->v[0][j] / rj;
+      uvec[2] += mj * w * st->v[1][j] / rj;
+      uvec[3] += mj * w * st->v[2][j] / rj;
+      uvec[4] += mj * w * st->e[j] / rj;
+      wnorm += mj * w / rj;
+      if (bj[0] != bi[0] || bj[1] != bi[1] || bj[2] != bi[2]) ks += mj * w * st->xi[j] / rj;
+    }
+    if (wnorm > 0) {
+      for (int k = 0; k < 5; k++) uvec[k] /= wnorm;
+      ks = ks / wnorm;
+    } else {
+      uvec[0] = 1; uvec[1] = uvec[2] = uvec[3] = uvec[4] = 0; ks = 0;
+    }
+    const double vdotn = uvec[1] * normal[0] + uvec[2] * normal[1] + uvec[3] * normal[2];
+    st->rho[g] = uvec[0];
+    for (int k = 0; k < 3; k++) st->v[k][g] = uvec[1 + k] - 2 * vdotn * normal[k];
+    st->e[g] = uvec[4];
+    st->xi[g] = ks;
+    orc_eos(cfg, st->rho[g], st->e[g], st->xi[g], &st->p[g], &st->c[g], &st->T[g], &st->gam[g], &st->cp[g]);
+  }
+  return 0;
+}
+
 int orc_step(const psph_config *cfg, orc_state *st) {
   int rc;
   double dt = orc_timestep(cfg, st);
@@ -1189,6 +1268,7 @@ int orc_step(const psph_config *cfg, orc_state *st) {
   if (cfg->dim == 3) {
     if ((rc = orc_remove_escaped(cfg, st))) return rc;
     if (orc_iferupt(cfg, st) && (rc = orc_inject(cfg, st, dt))) return rc;
+    if (cfg->wall_bc && (rc = orc_apply_bcond(cfg, st))) return rc;
     orc_sort_canonical(cfg, st);
   }
   orc_ifoutput(cfg, st);

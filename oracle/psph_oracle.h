@@ -90,6 +90,7 @@ int orc_vsmooth(const psph_config *cfg, orc_state *st);             /* smooth_ve
 int orc_update_pos(const psph_config *cfg, orc_state *st, double dt); /* update_pos.cc:34-168 */
 int orc_remove_escaped(const psph_config *cfg, orc_state *st);      /* update_pos.cc:171-463 */
 int orc_inject(const psph_config *cfg, orc_state *st, double dt);   /* setup_erupt.cc:389-670 */
+int orc_apply_bcond(const psph_config *cfg, orc_state *st);         /* bcond.cc:24-249 */
 int orc_step(const psph_config *cfg, orc_state *st);                /* main.cc:217-486, 676-751 */
 int orc_iferupt(const psph_config *cfg, const orc_state *st);       /* properties.h:126-135 */
 void orc_conservation(const orc_state *st, double out[7]);

@@ -1188,6 +1188,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
                                 double *grad_v, double *grad_sv, size_t n_cap) {
   if (!c || !out) return fail(PSPH_EINVAL, "null argument");
   const psph_config &k = c->cfg;
+  if (k.wall_bc) return fail(PSPH_ENOTSUP, "adjoint: the wall-ghost refresh (cfg.wall_bc) has no reverse; set it to 0");
   if (k.dim != 3) return fail(PSPH_ENOTSUP, "adjoint: built for the 3D plume only");
   if (nsteps < 1 || ckpt_every < 1) return fail(PSPH_EINVAL, "adjoint: nsteps and ckpt_every must be >= 1");
   if (c->tape.mode) return fail(PSPH_EINVAL, "adjoint: a recorded run is already in progress");
@@ -1343,6 +1344,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
 
 extern "C" int psph_replay_run(psph_ctx *c, int64_t nsteps) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (c->cfg.wall_bc) return fail(PSPH_ENOTSUP, "replay: the wall-ghost refresh (cfg.wall_bc) is not part of the recorded map");
   Tape &tp = c->tape;
   if (tp.mode) return fail(PSPH_EINVAL, "replay: a recorded run is in progress");
   if (nsteps < 0 || nsteps > (int64_t)tp.dt.size()) return fail(PSPH_EINVAL, "replay: %lld steps, %zu recorded", (long long)nsteps, tp.dt.size());

@@ -1327,6 +1327,137 @@ struct Bound : P {
 };
 
 // ---------------------------------------------------------------------------
+// Wall-ghost refresh: apply_bcond (bcond.cc:24-249) with the mirror images of
+// search_bnd_images (bnd_image.cc:33-254) for a flat ground at z = zg.  Each
+// wall ghost of the bucket layer that straddles the ground (bucket type
+// MIXED) takes the state at its image (x, y, 2 zg - z): Shepard sums over the
+// real and pressure-ghost particles j of the image's bucket and its 26
+// neighbours, m_j W(s, h_j) with s = (image - x_j) / h_j inside the in_support
+// box CUTOFF2 h_j, of rho, v / rho and e / rho over sum m_j W / rho_j; the
+// mass fraction sums the neighbour buckets only (bcond.cc:140-146); the
+// velocity is reflected in the ground plane and the EOS re-evaluated.  One
+// thread per particle; candidates come from the cell grid of the step's
+// neighbour search, with the reach widened by the largest displacement
+// since (positions have moved, the grid has not).  Only bc 2 entries are
+// written and only bc 100/99/1 entries read, so ghosts are independent.
+// ---------------------------------------------------------------------------
+struct BcondArgs {
+  const double *x, *y, *z, *h, *m;
+  double *rho, *vx, *vy, *vz, *e, *xi, *p, *c, *T, *gam, *cp;
+  const int32_t *bc;
+  const uint32_t *cs;
+  GridDesc g;
+  double reach, zg, del, cutoff2;
+  double ext_lo[3];
+  int nb[3], bzg;
+  CutTest cut;
+  EosConst eos;
+};
+
+__device__ inline bool bucket_of(const BcondArgs &a, const double *p, int *b) {
+  bool ok = true;
+  for (int k = 0; k < 3; k++) {
+    const double f = floor((p[k] - a.ext_lo[k]) / a.del);
+    ok = ok && f >= 0 && f < a.nb[k];
+    b[k] = ok ? (int)f : 0;
+  }
+  return ok;
+}
+
+__global__ void k_bcond(uint32_t n, BcondArgs a) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || a.bc[i] != 2) return;
+  const double coord[3] = {a.x[i], a.y[i], a.z[i]};
+  int bg[3];
+  if (!bucket_of(a, coord, bg) || bg[2] != a.bzg) return;
+  const double refc[3] = {coord[0], coord[1], 2 * a.zg - coord[2]};
+  double nrm[3], d = 0;
+  for (int k = 0; k < 3; k++) { nrm[k] = refc[k] - coord[k]; d += nrm[k] * nrm[k]; }
+  if (!(d > 0)) { nrm[0] = nrm[1] = 0; nrm[2] = 1; d = 1; }
+  const double sd = sqrt(d);
+  for (int k = 0; k < 3; k++) nrm[k] /= sd;
+  int bi[3];
+  if (!bucket_of(a, refc, bi)) return;
+  const int nn[3] = {a.g.nx, a.g.ny, a.g.nz};
+  int c0[3], c1[3];
+  for (int k = 0; k < 3; k++) {
+    if (a.g.inv_e > 0) {
+      c0[k] = cell_of(refc[k] - a.reach, a.g.lo[k], a.g.inv_e, nn[k]);
+      c1[k] = cell_of(refc[k] + a.reach, a.g.lo[k], a.g.inv_e, nn[k]);
+    } else {
+      c0[k] = c1[k] = 0;
+    }
+  }
+  double u0 = 0, u1 = 0, u2 = 0, u3 = 0, u4 = 0, wnorm = 0, ks = 0;
+  for (int cz = c0[2]; cz <= c1[2]; cz++)
+    for (int cy = c0[1]; cy <= c1[1]; cy++)
+      for (int cx = c0[0]; cx <= c1[0]; cx++) {
+        const uint32_t cell = ((uint32_t)cz * a.g.ny + cy) * a.g.nx + cx;
+        for (uint32_t j = a.cs[cell]; j < a.cs[cell + 1]; j++) {
+          const int b = a.bc[j];
+          if (!(b == 100 || b == 99 || b == 1)) continue;           // contr_image (particle.h:562-565)
+          const double pj[3] = {a.x[j], a.y[j], a.z[j]};
+          int bj[3];
+          if (!bucket_of(a, pj, bj)) continue;
+          if (abs(bj[0] - bi[0]) > 1 || abs(bj[1] - bi[1]) > 1 || abs(bj[2] - bi[2]) > 1) continue;
+          const double hj = a.h[j];
+          const double dx[3] = {refc[0] - pj[0], refc[1] - pj[1], refc[2] - pj[2]};
+          const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
+          if (amax > a.cutoff2 * hj || a.cut.outside(amax, hj)) continue;   // in_support; W = 0 outside the box
+          const double ih = 1.0 / hj;
+          double expt = 0;
+          for (int k = 0; k < 3; k++) { const double sk = dx[k] * ih; expt += sk * sk; }
+          const double nk = kInvSqrtPi * ih;
+          const double w = nk * nk * nk * exp(-expt);
+          const double mj = a.m[j], rj = a.rho[j];
+          const double mw = mj * w;
+          u0 += mw;
+          u1 += mw * a.vx[j] / rj;
+          u2 += mw * a.vy[j] / rj;
+          u3 += mw * a.vz[j] / rj;
+          u4 += mw * a.e[j] / rj;
+          wnorm += mw / rj;
+          if (bj[0] != bi[0] || bj[1] != bi[1] || bj[2] != bi[2]) ks += mw * a.xi[j] / rj;
+        }
+      }
+  if (wnorm > 0) {
+    u0 /= wnorm; u1 /= wnorm; u2 /= wnorm; u3 /= wnorm; u4 /= wnorm;
+    ks = ks / wnorm;
+  } else {
+    u0 = 1; u1 = u2 = u3 = u4 = 0; ks = 0;
+  }
+  const double vdotn = u1 * nrm[0] + u2 * nrm[1] + u3 * nrm[2];
+  a.rho[i] = u0;
+  a.vx[i] = u1 - 2 * vdotn * nrm[0];
+  a.vy[i] = u2 - 2 * vdotn * nrm[1];
+  a.vz[i] = u3 - 2 * vdotn * nrm[2];
+  a.e[i] = u4;
+  a.xi[i] = ks;
+  double p, c, T, gam, cp = a.cp[i];
+  eos_dev(a.eos, u0, u4, ks, p, c, T, gam, &cp);
+  a.p[i] = p; a.c[i] = c; a.T[i] = T; a.gam[i] = gam; a.cp[i] = cp;
+}
+
+// largest per-axis speed of the particles update_pos moves (reals with the
+// filtered velocity under LANS, erupt ghosts with v)
+__global__ __launch_bounds__(256) void k_speed_max(uint32_t n, const int32_t *bc, int lans, const double *vx,
+                                                   const double *vy, const double *vz, const double *svx,
+                                                   const double *svy, const double *svz, unsigned long long *out) {
+  double v = 0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int b = bc[i];
+    if (b == 100 || b == 99) {
+      const bool s = lans != 0;
+      v = fmax(v, fmax(fabs(s ? svx[i] : vx[i]), fmax(fabs(s ? svy[i] : vy[i]), fabs(s ? svz[i] : vz[i]))));
+    } else if (b == 0) {
+      v = fmax(v, fmax(fabs(vx[i]), fmax(fabs(vy[i]), fabs(vz[i]))));
+    }
+  }
+  v = block_ext<true>(v);
+  if (threadIdx.x == 0 && v > 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+// ---------------------------------------------------------------------------
 // Position update + escape (update_pos.cc:34-168, bucket.cc:435-452)
 // ---------------------------------------------------------------------------
 // Schedule of a recorded run: tmode 1 records, by particle id, the tape step
@@ -2273,6 +2404,39 @@ extern "C" int psph_inject(psph_ctx *c, double dt, int64_t *added) {
   return 0;
 }
 
+// wall-ghost refresh over the cell grid of the last neighbour search; disp:
+// the largest per-axis displacement of any particle since that search
+static int apply_bcond_grid(psph_ctx *c, double disp) {
+  const psph_config &k = c->cfg;
+  if (k.dim != 3 || !c->n) return 0;
+  const double fz = (k.dom_lo[2] - k.ext_lo[2]) / k.bucket_edge;
+  if (fz == std::floor(fz)) return 0;            // ground on a bucket face: no MIXED bucket
+  BcondArgs a{};
+  a.x = c->f.x; a.y = c->f.y; a.z = c->f.z; a.h = c->f.h; a.m = c->f.m;
+  a.rho = c->f.rho; a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.e = c->f.e; a.xi = c->f.xi;
+  a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.gam = c->f.gam; a.cp = c->f.cp;
+  a.bc = c->f.bc; a.cs = c->cs; a.g = c->grid;
+  const double hm = std::max(c->h_max, k.ratio_sml_dx * k.h_air);
+  a.reach = k.cutoff * hm * (1 + 1e-9) + disp * (1 + 1e-9);
+  a.zg = k.dom_lo[2]; a.del = k.bucket_edge; a.cutoff2 = k.cutoff2;
+  for (int d = 0; d < 3; d++) {
+    a.ext_lo[d] = k.ext_lo[d];
+    a.nb[d] = (int)std::llround((k.ext_hi[d] - k.ext_lo[d]) / k.bucket_edge);
+  }
+  a.bzg = (int)std::floor(fz);
+  a.cut = cut_test(k.cutoff);
+  a.eos = eos_const(k);
+  hipLaunchKernelGGL(k_bcond, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, a);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+extern "C" int psph_apply_bcond(psph_ctx *c) {
+  if (!c) return fail(PSPH_EINVAL, "null ctx");
+  if (!c->grid_valid) PSPH_TRY(psph_neighbors(c));
+  return apply_bcond_grid(c, 0.0);
+}
+
 extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   const psph_config &k = c->cfg;
@@ -2319,6 +2483,19 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
     c->rec->inj_lo = id0;
     c->rec->inj_hi = c->next_pid;
     c->rec->inj_vel.assign(c->inj.vz.begin(), c->inj.vz.end());
+  }
+  if (k.wall_bc && k.dim == 3 && c->n) {
+    // apply_bcond at the end of the step (main.cc:468): the grid is the one of
+    // this step's neighbour search; particles have moved by at most dt vmax
+    hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 1, 0ull, 1);
+    hipLaunchKernelGGL(k_speed_max, red_blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, c->f.bc, k.lans, c->f.vx,
+                       c->f.vy, c->f.vz, c->f.svx, c->f.svy, c->f.svz, c->d_u64 + 1);
+    unsigned long long bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, c->d_u64 + 1, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double vmax;
+    std::memcpy(&vmax, &bits, 8);
+    PSPH_TRY(apply_bcond_grid(c, std::fabs(dt) * vmax));
   }
   if (tp.mode) tp.k++;
   // TimeProps::ifoutput (properties.h:104-120)

@@ -38,7 +38,7 @@ class PsphError(RuntimeError):
 
 _INT_FIELDS = ["dim", "precision", "kernel", "lans", "energy_smooth", "pghost_contrib", "list_all",
                "num_loop", "sml_update_int", "adapt_at_setup", "vent_profile", "max_steps",
-               "num_erupt_particles", "p_table_size"]
+               "num_erupt_particles", "p_table_size", "wall_bc"]
 _DBL_FIELDS = ["cutoff", "cutoff2", "list_factor", "alpha", "beta", "ata", "eps_half", "prandtl",
                "e_smooth_ratio", "heat_scale", "g", "cfl", "thresh", "c_smooth", "eta",
                "ratio_sml_dx", "msfrc_thresh", "ng0", "cvs", "cvg", "cva", "rg", "ra", "gamma_p",
@@ -50,7 +50,7 @@ _DBL_TAIL = ["t_end", "t_output", "erupt_start", "erupt_end", "vent_r", "vent_v"
 
 class Config(C.Structure):
     """psph_config (include/psph.h)."""
-    _fields_ = ([(f, C.c_int32) for f in _INT_FIELDS] + [("reserved_i", C.c_int32 * 2)]
+    _fields_ = ([(f, C.c_int32) for f in _INT_FIELDS] + [("reserved_i", C.c_int32 * 1)]
                 + [(f, C.c_double) for f in _DBL_FIELDS]
                 + [("dom_lo", C.c_double * 3), ("dom_hi", C.c_double * 3),
                    ("ext_lo", C.c_double * 3), ("ext_hi", C.c_double * 3)]
@@ -178,6 +178,7 @@ def lib() -> C.CDLL:
         "psph_vsmooth": (C.c_int, [P]),
         "psph_update_pos": (C.c_int, [P, C.c_double]),
         "psph_inject": (C.c_int, [P, C.c_double, C.POINTER(C.c_int64)]),
+        "psph_apply_bcond": (C.c_int, [P]),
         "psph_step": (C.c_int, [P, C.POINTER(StepOut)]),
         "psph_run": (C.c_int, [P, C.c_int64, C.POINTER(StepOut)]),
         "psph_get_neighbors": (C.c_int, [P, C.POINTER(C.c_int64), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t]),
@@ -383,6 +384,9 @@ class Solver:
         n = C.c_int64()
         check(lib().psph_inject(self._ctx, dt, C.byref(n)))
         return n.value
+
+    def apply_bcond(self):
+        check(lib().psph_apply_bcond(self._ctx))
 
     def step(self) -> StepOut:
         o = StepOut()

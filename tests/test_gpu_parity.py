@@ -86,6 +86,29 @@ def test_plume_steps(nsteps, adapt):
         assert abs(G.diag_height() - S.diag_height()) <= 1e-10 * max(1.0, abs(S.diag_height()))
 
 
+def test_plume_steps_wall_bc():
+    """The reference's loop with the wall-ghost refresh (apply_bcond before the
+    loop and after every step, main.cc:196, 468), adaptive h at step 3."""
+    cfg, met, P = small_plume(sml_update_int=3)
+    cfg.wall_bc = 1
+    say(f"plume with wall refresh, 6 steps: {len(P)} particles")
+    S = oracle.State(cfg, P)
+    S.apply_bcond()
+    with psph.Solver(cfg, len(P) + 20000) as G:
+        G.load(P)
+        G.apply_bcond()
+        A0, B0 = G.read(), S.particles()
+        walls = B0["bc"] == 2
+        assert not np.array_equal(B0["rho"][walls], P["rho"][P["bc"] == 2])   # the refresh changed them
+        errs = compare(A0, B0, FLOAT_FIELDS_3D, REL_TOL_FP64)
+        say(f"after the set-up refresh: max rel err {max(errs.values()):.3e}")
+        for s in range(6):
+            S.step()
+            G.step()
+        errs = compare(G.read(), S.particles(), FLOAT_FIELDS_3D, REL_TOL_FP64)
+        say(f"after 6 steps: max rel err {max(errs.values()):.3e}")
+
+
 def test_sod_steps():
     """Config 1 (1D Sod, plain SPH): adaptive h at set-up and every step."""
     cfg, P = sod()
