@@ -184,6 +184,8 @@ def run_form(cfg, P, args, form):
     cap = len(P) + 400 * (args.warmup + args.steps) + 100000
     G = psph.Solver(cfg, cap)
     G.load(P)
+    if cfg.wall_bc:
+        G.apply_bcond()
     t0 = time.perf_counter()
     for s in range(args.warmup):
         G.step()
@@ -330,6 +332,8 @@ def main():
     ap.add_argument("--adjoint-fd", type=int, default=3, help="vent parameters checked by central differences")
     ap.add_argument("--adjoint-short", type=int, default=20, help="horizon of the short FD check")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--wall-bc", action="store_true",
+                    help="forward forms with the wall-ghost refresh after every step (as the reference's loop)")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
     ap.add_argument("--strong", action="store_true", help="also run the 16M strong-plume leg (minutes)")
     ap.add_argument("--strong-only", action="store_true", help="run only the strong-plume leg")
@@ -364,6 +368,7 @@ def main():
         return
 
     base = weak_plume_config(args.particles)
+    base.wall_bc = 1 if args.wall_bc else 0
     met = psph.meteo_standard()
     t0 = time.perf_counter()
     P = psph.gen_plume(base, met, args.seed, 0.01)
@@ -455,7 +460,8 @@ def main():
                                f"{ {'fp64': 'FP64 form', 'fp32-pair': 'FP32-pair form', 'fp32-pure': 'FP32-pure form'}[args.precision]}",
                    "particles_real": n_real, "particles_total": n_total_end,
                    "spacing_m": cfg.h_air, "vent": "r=54 m, 135 m/s, 1053 K, gas fraction 0.03",
-                   "kernel": "gaussian-box", "steps_from": step - args.steps, "sim_time_s": t_sim},
+                   "kernel": "gaussian-box", "wall_ghost_refresh": bool(args.wall_bc),
+                   "steps_from": step - args.steps, "sim_time_s": t_sim},
         "roofline": roofline, "roofline_valu": roofline_valu, "cpu_baseline": cpu,
         "forms": compare,
         "adjoint": adjoint,

@@ -1335,11 +1335,11 @@ struct Bound : P {
 // neighbours, m_j W(s, h_j) with s = (image - x_j) / h_j inside the in_support
 // box CUTOFF2 h_j, of rho, v / rho and e / rho over sum m_j W / rho_j; the
 // mass fraction sums the neighbour buckets only (bcond.cc:140-146); the
-// velocity is reflected in the ground plane and the EOS re-evaluated.  One
-// thread per particle; candidates come from the cell grid of the step's
-// neighbour search, with the reach widened by the largest displacement
-// since (positions have moved, the grid has not).  Only bc 2 entries are
-// written and only bc 100/99/1 entries read, so ghosts are independent.
+// velocity is reflected in the ground plane and the EOS re-evaluated.
+// Candidates come from the cell grid of the step's neighbour search, with the
+// reach widened by the largest displacement since (positions have moved, the
+// grid has not).  Only bc 2 entries are written and only bc 100/99/1 entries
+// read, so ghosts are independent.
 // ---------------------------------------------------------------------------
 struct BcondArgs {
   const double *x, *y, *z, *h, *m;
@@ -1364,22 +1364,37 @@ __device__ inline bool bucket_of(const BcondArgs &a, const double *p, int *b) {
   return ok;
 }
 
-__global__ void k_bcond(uint32_t n, BcondArgs a) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || a.bc[i] != 2) return;
-  const double coord[3] = {a.x[i], a.y[i], a.z[i]};
-  int bg[3];
-  if (!bucket_of(a, coord, bg) || bg[2] != a.bzg) return;
-  const double refc[3] = {coord[0], coord[1], 2 * a.zg - coord[2]};
-  double nrm[3], d = 0;
-  for (int k = 0; k < 3; k++) { nrm[k] = refc[k] - coord[k]; d += nrm[k] * nrm[k]; }
-  if (!(d > 0)) { nrm[0] = nrm[1] = 0; nrm[2] = 1; d = 1; }
-  const double sd = sqrt(d);
-  for (int k = 0; k < 3; k++) nrm[k] /= sd;
-  int bi[3];
-  if (!bucket_of(a, refc, bi)) return;
+// One wavefront per 64 consecutive particles (in cell order, so a wave's
+// ghosts and their images are compact): the wave stages the candidate cells'
+// particles 64 at a time into LDS, with their bucket indices and 1/rho; each
+// lane holding an imaged ghost marks the staged candidates that reach its
+// image, then walks only those, in index order.
+__global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
+  __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], smw[kWave], sir[kWave];
+  __shared__ double svx[kWave], svy[kWave], svz[kWave], se[kWave], sxi[kWave];
+  __shared__ int sbk[3][kWave];
+  const int lane = threadIdx.x;
+  const uint32_t i = blockIdx.x * kWave + lane;
+  double coord[3] = {0, 0, 0}, refc[3] = {0, 0, 0};
+  int bi[3] = {0, 0, 0};
+  bool act = false;
+  if (i < n && a.bc[i] == 2) {
+    coord[0] = a.x[i]; coord[1] = a.y[i]; coord[2] = a.z[i];
+    int bg[3];
+    if (bucket_of(a, coord, bg) && bg[2] == a.bzg) {
+      refc[0] = coord[0]; refc[1] = coord[1]; refc[2] = 2 * a.zg - coord[2];
+      act = bucket_of(a, refc, bi);
+    }
+  }
+  if (!__any(act)) return;
+  // Each active lane's cells within reach of its image form a box, walked
+  // in increasing cell index with a cursor; the wave visits the union of the
+  // boxes in that order (the smallest pending cell of any lane next).  A
+  // lane's hits therefore come cell by cell in increasing index, as in the
+  // bounding-box walk, without the cells between far-apart images (a wave's
+  // ghosts can straddle two rows of cells).
   const int nn[3] = {a.g.nx, a.g.ny, a.g.nz};
-  int c0[3], c1[3];
+  int c0[3], c1[3], cur[3];
   for (int k = 0; k < 3; k++) {
     if (a.g.inv_e > 0) {
       c0[k] = cell_of(refc[k] - a.reach, a.g.lo[k], a.g.inv_e, nn[k]);
@@ -1387,39 +1402,89 @@ __global__ void k_bcond(uint32_t n, BcondArgs a) {
     } else {
       c0[k] = c1[k] = 0;
     }
+    cur[k] = c0[k];
   }
+  constexpr uint32_t kDone = 0xffffffffu;
+  uint32_t mine = act ? ((uint32_t)cur[2] * a.g.ny + cur[1]) * a.g.nx + cur[0] : kDone;
   double u0 = 0, u1 = 0, u2 = 0, u3 = 0, u4 = 0, wnorm = 0, ks = 0;
-  for (int cz = c0[2]; cz <= c1[2]; cz++)
-    for (int cy = c0[1]; cy <= c1[1]; cy++)
-      for (int cx = c0[0]; cx <= c1[0]; cx++) {
-        const uint32_t cell = ((uint32_t)cz * a.g.ny + cy) * a.g.nx + cx;
-        for (uint32_t j = a.cs[cell]; j < a.cs[cell + 1]; j++) {
-          const int b = a.bc[j];
-          if (!(b == 100 || b == 99 || b == 1)) continue;           // contr_image (particle.h:562-565)
-          const double pj[3] = {a.x[j], a.y[j], a.z[j]};
-          int bj[3];
-          if (!bucket_of(a, pj, bj)) continue;
-          if (abs(bj[0] - bi[0]) > 1 || abs(bj[1] - bi[1]) > 1 || abs(bj[2] - bi[2]) > 1) continue;
-          const double hj = a.h[j];
-          const double dx[3] = {refc[0] - pj[0], refc[1] - pj[1], refc[2] - pj[2]};
-          const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-          if (amax > a.cutoff2 * hj || a.cut.outside(amax, hj)) continue;   // in_support; W = 0 outside the box
+  for (;;) {
+    uint32_t cell = mine;
+    for (int o = 32; o > 0; o >>= 1) cell = min(cell, (uint32_t)__shfl_xor((int)cell, o));
+    if (cell == kDone) break;
+    const bool here = mine == cell;
+    if (here) {
+      if (++cur[0] > c1[0]) {
+        cur[0] = c0[0];
+        if (++cur[1] > c1[1]) {
+          cur[1] = c0[1];
+          ++cur[2];
+        }
+      }
+      mine = cur[2] > c1[2] ? kDone : ((uint32_t)cur[2] * a.g.ny + cur[1]) * a.g.nx + cur[0];
+    }
+    const uint32_t lo = a.cs[cell], hi = a.cs[cell + 1];
+    for (uint32_t base = lo; base < hi; base += kWave) {
+      const uint32_t j = base + lane;
+      bool ok = false;
+      if (j < hi) {
+        const int b = a.bc[j];
+        const double pj[3] = {a.x[j], a.y[j], a.z[j]};
+        int bj[3];
+        ok = (b == 100 || b == 99 || b == 1) && bucket_of(a, pj, bj);   // contr_image (particle.h:562-565)
+        if (ok) {
+          sx[lane] = pj[0]; sy[lane] = pj[1]; sz[lane] = pj[2];
+          sh[lane] = a.h[j]; smw[lane] = a.m[j]; sir[lane] = 1.0 / a.rho[j];
+          svx[lane] = a.vx[j]; svy[lane] = a.vy[j]; svz[lane] = a.vz[j];
+          se[lane] = a.e[j]; sxi[lane] = a.xi[j];
+          for (int k = 0; k < 3; k++) sbk[k][lane] = bj[k];
+        }
+      }
+      if (!ok) sbk[0][lane] = -1000;            // never within one bucket of an image
+      __syncthreads();
+      const uint32_t cnt = min((uint32_t)kWave, hi - base);
+      if (here) {
+        // phase 1: which staged candidates reach this lane's image
+        uint64_t mask = 0;
+        for (uint32_t q = 0; q < cnt; q++) {
+          const bool near = abs(sbk[0][q] - bi[0]) <= 1 && abs(sbk[1][q] - bi[1]) <= 1 &&
+                            abs(sbk[2][q] - bi[2]) <= 1;
+          const double hj = sh[q];
+          const double amax = fmax(fabs(refc[0] - sx[q]), fmax(fabs(refc[1] - sy[q]), fabs(refc[2] - sz[q])));
+          // in_support; W = 0 outside the kernel box
+          const bool in = near & (amax <= a.cutoff2 * hj) & !a.cut.outside(amax, hj);
+          mask |= (uint64_t)in << q;
+        }
+        // phase 2: the lane's own hits, in index order
+        while (mask) {
+          const uint32_t q = (uint32_t)__builtin_ctzll(mask);
+          mask &= mask - 1;
+          const double hj = sh[q];
+          const double dx[3] = {refc[0] - sx[q], refc[1] - sy[q], refc[2] - sz[q]};
           const double ih = 1.0 / hj;
           double expt = 0;
           for (int k = 0; k < 3; k++) { const double sk = dx[k] * ih; expt += sk * sk; }
           const double nk = kInvSqrtPi * ih;
           const double w = nk * nk * nk * exp(-expt);
-          const double mj = a.m[j], rj = a.rho[j];
-          const double mw = mj * w;
+          const double ir = sir[q];
+          const double mw = smw[q] * w;
           u0 += mw;
-          u1 += mw * a.vx[j] / rj;
-          u2 += mw * a.vy[j] / rj;
-          u3 += mw * a.vz[j] / rj;
-          u4 += mw * a.e[j] / rj;
-          wnorm += mw / rj;
-          if (bj[0] != bi[0] || bj[1] != bi[1] || bj[2] != bi[2]) ks += mw * a.xi[j] / rj;
+          u1 += mw * svx[q] * ir;
+          u2 += mw * svy[q] * ir;
+          u3 += mw * svz[q] * ir;
+          u4 += mw * se[q] * ir;
+          wnorm += mw * ir;
+          if (sbk[0][q] != bi[0] || sbk[1][q] != bi[1] || sbk[2][q] != bi[2]) ks += mw * sxi[q] * ir;
         }
       }
+      __syncthreads();
+    }
+  }
+  if (!act) return;
+  double nrm[3], d = 0;
+  for (int k = 0; k < 3; k++) { nrm[k] = refc[k] - coord[k]; d += nrm[k] * nrm[k]; }
+  if (!(d > 0)) { nrm[0] = nrm[1] = 0; nrm[2] = 1; d = 1; }
+  const double sd = sqrt(d);
+  for (int k = 0; k < 3; k++) nrm[k] /= sd;
   if (wnorm > 0) {
     u0 /= wnorm; u1 /= wnorm; u2 /= wnorm; u3 /= wnorm; u4 /= wnorm;
     ks = ks / wnorm;
@@ -2426,7 +2491,7 @@ static int apply_bcond_grid(psph_ctx *c, double disp) {
   a.bzg = (int)std::floor(fz);
   a.cut = cut_test(k.cutoff);
   a.eos = eos_const(k);
-  hipLaunchKernelGGL(k_bcond, blocks(c->n), 256, 0, c->stream, (uint32_t)c->n, a);
+  hipLaunchKernelGGL(k_bcond, blocks(c->n, kWave), kWave, 0, c->stream, (uint32_t)c->n, a);
   HIP_TRY(hipGetLastError());
   return 0;
 }
