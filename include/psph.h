@@ -144,6 +144,9 @@ typedef struct psph_step_out {
   int64_t n_injected;       /* particles added by the inlet this step          */
   int64_t n_removed;        /* real particles that escaped the domain          */
   int64_t n_pairs;          /* kernel-support pairs in the RHS pass            */
+  int64_t output;           /* 1: the step reached an output time, the point
+                               where the reference's loop writes its files
+                               (TimeProps::ifoutput, main.cc:473-477)         */
 } psph_step_out;
 
 typedef struct psph_ctx psph_ctx;
@@ -209,6 +212,8 @@ int psph_load(psph_ctx *ctx, const psph_soa *host, size_t n, double t0, int64_t 
 int psph_read(psph_ctx *ctx, psph_soa *host, size_t cap, size_t *n_out);
 int psph_count(psph_ctx *ctx, size_t *n_total, size_t *n_real);
 int psph_time(psph_ctx *ctx, double *time, int64_t *step);
+/* The configuration the context runs (psph_create's, as completed there). */
+int psph_get_config(psph_ctx *ctx, psph_config *cfg);
 
 /* ---- stages (reference granularity) ------------------------------------ */
 /* dt = CFL * min h/c over need_neigh particles (timestep.cc:94-135, 1D

@@ -1901,6 +1901,12 @@ extern "C" int psph_count(psph_ctx *c, size_t *n_total, size_t *n_real) {
   return 0;
 }
 
+extern "C" int psph_get_config(psph_ctx *c, psph_config *cfg) {
+  if (!c || !cfg) return fail(PSPH_EINVAL, "null argument");
+  *cfg = c->cfg;
+  return 0;
+}
+
 extern "C" int psph_time(psph_ctx *c, double *time, int64_t *step) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   if (time) *time = c->time;
@@ -2564,9 +2570,11 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
   }
   if (tp.mode) tp.k++;
   // TimeProps::ifoutput (properties.h:104-120)
+  bool output = false;
   if (c->time >= c->next_output || c->time == k.t_end) {
     c->ioutput++;
     c->next_output = ((c->ioutput + 1) * k.t_output) / 1.0;
+    output = true;
   }
   if (out) {
     out->dt = dt; out->time = c->time; out->step = c->step;
@@ -2574,6 +2582,7 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
     out->n_real = -1;
     out->n_injected = added; out->n_removed = c->last_removed;
     out->n_pairs = (int64_t)c->last_pairs;
+    out->output = output;
   }
   return 0;
 }

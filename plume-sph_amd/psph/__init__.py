@@ -81,7 +81,8 @@ class SoA(C.Structure):
 class StepOut(C.Structure):
     _fields_ = [("dt", C.c_double), ("time", C.c_double), ("step", C.c_int64),
                 ("n_total", C.c_int64), ("n_real", C.c_int64), ("n_injected", C.c_int64),
-                ("n_removed", C.c_int64), ("n_pairs", C.c_int64)]
+                ("n_removed", C.c_int64), ("n_pairs", C.c_int64),
+                ("output", C.c_int64)]
 
 
 class AdjointOut(C.Structure):
@@ -169,6 +170,7 @@ def lib() -> C.CDLL:
         "psph_read": (C.c_int, [P, SOA, C.c_size_t, C.POINTER(C.c_size_t)]),
         "psph_count": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "psph_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "psph_get_config": (C.c_int, [P, CFG]),
         "psph_timestep": (C.c_int, [P, C.POINTER(C.c_double)]),
         "psph_advance_time": (C.c_int, [P, C.POINTER(C.c_double)]),
         "psph_neighbors": (C.c_int, [P]),
@@ -332,6 +334,11 @@ class Solver:
     def load(self, P: Particles, t0: float = 0.0, step0: int = 0):
         s = P.soa()
         check(lib().psph_load(self._ctx, C.byref(s), len(P), t0, step0))
+
+    def config(self) -> Config:
+        c = Config()
+        check(lib().psph_get_config(self._ctx, C.byref(c)))
+        return c
 
     def read(self) -> Particles:
         n = C.c_size_t()

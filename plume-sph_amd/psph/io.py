@@ -1,0 +1,107 @@
+"""psph.io -- H5Part and CSV output of the particle state over the host
+library libpsph_io.so (include/psph_io.h), loaded with ctypes.
+
+The reference writes one HDF5 group ``Step#k`` per output into
+``pvplot%03d.h5part`` (h5part_writer.cc:40-271), creating the file at the
+first output of a run and appending after (TimeProps::ifstart), with a
+per-process output counter; :class:`H5PartWriter` keeps that counter for a
+:class:`psph.Solver`.  The 1D driver also appends a CSV table
+(h5part_writer.cc:448-489) near a comparison time (main.cc:743-750).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import PsphError, Particles, Solver, check, lib
+
+IO_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpsph_io.so")
+
+REAL, SHOW, ALL = 0, 1, 2                       # PSPH_H5_REAL / SHOW / ALL
+INVOLVED, PID, MASS, SML = 1, 2, 4, 8           # PSPH_H5_* extra datasets
+
+INT_DATASETS = {"ID", "Involved", "myprocess", "phase", "bctp", "guest"}
+DATASETS = ["ID", "x", "y", "z", "Rho", "Vx", "Vy", "Vz", "engr", "mssfrc", "prss", "phase", "bctp", "guest"]
+
+_io = None
+
+
+def io_lib():
+    global _io
+    if _io is not None:
+        return _io
+    lib()                                        # libpsph.so first: the output library links it
+    if not os.path.exists(IO_PATH):
+        raise PsphError(-7, f"{IO_PATH} is not built: run plume-sph_amd/build.sh")
+    L = C.CDLL(IO_PATH)
+    P, SOA = C.c_void_p, C.c_void_p
+    sig = {
+        "psph_h5part_write_soa": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, SOA, C.c_size_t, C.c_int32,
+                                            C.c_int32, C.c_int32, C.c_int32]),
+        "psph_write_h5part": (C.c_int, [P, C.c_char_p, C.c_int, C.c_int64, C.c_int32, C.c_int32]),
+        "psph_h5part_steps": (C.c_int, [C.c_char_p, C.POINTER(C.c_int64)]),
+        "psph_h5part_read": (C.c_int, [C.c_char_p, C.c_int64, C.c_char_p, C.c_int32, C.c_void_p, C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
+        "psph_csv_write_soa": (C.c_int, [C.c_char_p, SOA, C.c_size_t]),
+        "psph_write_csv": (C.c_int, [P, C.c_char_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype, f.argtypes = res, args
+    _io = L
+    return L
+
+
+def h5part_write(path: str, P: Particles, group: int, create: bool, dim: int = 3, which: int = REAL,
+                 extras: int = 0, rank: int = 0):
+    """Step#<group> of host particles P (in the order the file should hold)."""
+    s = P.soa()
+    check(io_lib().psph_h5part_write_soa(os.fsencode(path), int(create), group, C.addressof(s), len(P), dim,
+                                         which, extras, rank))
+
+
+def h5part_steps(path: str) -> int:
+    n = C.c_int64()
+    check(io_lib().psph_h5part_steps(os.fsencode(path), C.byref(n)))
+    return n.value
+
+
+def h5part_read(path: str, group: int, name: str) -> np.ndarray:
+    """Dataset `name` of Step#<group> (int32 for the integer datasets, float64 otherwise)."""
+    is_int = name in INT_DATASETS
+    n = C.c_size_t()
+    check(io_lib().psph_h5part_read(os.fsencode(path), group, name.encode(), int(is_int), None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.int32 if is_int else np.float64)
+    check(io_lib().psph_h5part_read(os.fsencode(path), group, name.encode(), int(is_int),
+                                    out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+    return out
+
+
+def h5part_step(path: str, group: int, names=DATASETS) -> dict:
+    return {k: h5part_read(path, group, k) for k in names}
+
+
+def csv_write(path: str, P: Particles):
+    s = P.soa()
+    check(io_lib().psph_csv_write_soa(os.fsencode(path), C.addressof(s), len(P)))
+
+
+def write_csv(solver: Solver, path: str):
+    check(io_lib().psph_write_csv(solver._ctx, os.fsencode(path)))
+
+
+class H5PartWriter:
+    """Outputs of one run into one H5Part file: Step#0 created with the file
+    (or truncating it), later steps appended (write_h5part's static counter
+    and ifstart test, h5part_writer.cc:65, 75-78, 128-129)."""
+
+    def __init__(self, path: str, which: int = REAL, extras: int = 0):
+        self.path, self.which, self.extras = path, which, extras
+        self.count = 0
+
+    def write(self, solver: Solver):
+        check(io_lib().psph_write_h5part(solver._ctx, os.fsencode(self.path), int(self.count == 0), self.count,
+                                         self.which, self.extras))
+        self.count += 1

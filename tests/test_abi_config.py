@@ -13,8 +13,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
 
 
-def header_functions():
-    src = open(os.path.join(ROOT, "include", "psph.h")).read()
+def header_functions(name="psph.h"):
+    src = open(os.path.join(ROOT, "include", name)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(psph_[a-z0-9_]+)\s*\(", src)))
 
@@ -23,6 +23,15 @@ def test_library_exports_every_declared_symbol():
     L = psph.lib()
     names = header_functions()
     assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_io_library_exports_every_declared_symbol():
+    import psph.io as pio
+    L = pio.io_lib()
+    names = header_functions("psph_io.h")
+    assert len(names) >= 6
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
