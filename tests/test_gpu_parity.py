@@ -126,3 +126,16 @@ def test_sod_steps():
             G.step()
         errs = compare(G.read(), S.particles(), FLOAT_FIELDS_1D, REL_TOL_FP64)
         say(f"sod max rel err {max(errs.values()):.3e}")
+
+
+def test_inlet_past_capacity_is_an_error():
+    """The inlet adds the vent's new particles each step; a context sized too
+    small for them reports ECAPACITY before any device work of the inlet."""
+    cfg, met, P = small_plume(sml_update_int=50)
+    say(f"inlet past capacity: {len(P)} particles, capacity {len(P) + 8}")
+    with psph.Solver(cfg, len(P) + 8) as G:
+        G.load(P)
+        with pytest.raises(psph.PsphError) as e:
+            for s in range(4):
+                G.step()
+        assert e.value.code == -4 and "inject" in str(e.value)
