@@ -36,6 +36,22 @@ def small_plume(seed=20261202, jitter=0.01, **kw):
     return cfg, met, P
 
 
+def weak_plume_config(n_real=4.0e6, lx=8000.0, ly=8000.0, lz=12000.0):
+    """The weak dusty-gas plume of the throughput measurement: 8 x 8 x 12 km
+    with n_real real particles (spacing (V/N)^(1/3)), vent r = 54 m, 135 m/s,
+    1053 K, erupted-material gas fraction 0.03, p(0) = 101325 Pa."""
+    cfg = psph.default_config(3)
+    cfg.h_air = (lx * ly * lz / n_real) ** (1.0 / 3.0)
+    for k, (lo, hi) in enumerate([(-lx / 2, lx / 2), (-ly / 2, ly / 2), (0.0, lz)]):
+        cfg.dom_lo[k] = lo
+        cfg.dom_hi[k] = hi
+    cfg.vent_z = 0.0
+    cfg.vent_r, cfg.vent_v, cfg.vent_T, cfg.ng0 = 54.0, 135.0, 1053.0, 0.03
+    cfg.vent_p = 101325.0
+    psph.finalize_config(cfg)
+    return cfg
+
+
 def sod_config(n_left=800, n_right=100, eta=1.3, t_end=0.2):
     """Config 1: Sod tube on [-0.5, 0.5], equal-mass particles, plain SPH."""
     cfg = psph.default_config(1)
