@@ -206,11 +206,26 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_count(AdjCommon pc, uint32_
   if (threadIdx.x == 0) vcount[chunk] = cnt;
 }
 
+// The scan is transposed as the forward one (k_visit_masks): lane l holds
+// candidate 2 (l mod 32) + l / 32 of the tile, the wave loops over the
+// chunk's rows, and the two ballots of a row's role tests are, in their low
+// and high halves, the masks of the row's two sub-lanes.  A row is skipped
+// for a tile when its distance to the tile's box exceeds cut * max(h_i,
+// h_max of the tile): both roles' kernel boxes, cut * max(h_s, h_ij) with h_s
+// the summing particle's h, lie within that reach for every candidate of the
+// tile, and the rounded pair quantities stay on the same side of it (a
+// rounded mean lies between its operands; rounding is monotonic), so the
+// masks are exactly those of testing every row.
+struct AdjVisitRow {
+  double x, y, z, h, lf, supp;   // position, h, list box, in_support box
+  uint32_t refb;
+  int tgt;                       // need_neigh: the row sums (own role)
+};
+
 __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double cutx, const uint32_t *voff,
                                                            uint32_t *vnum, uint2 *vent, uint64_t *vmask) {
-  __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], slf[kWave];
-  __shared__ uint32_t srefb[kWave];
-  __shared__ int sbc[kWave];
+  static_assert(kSub == 2, "the transposed scan maps the ballot halves onto two sub-lanes");
+  __shared__ AdjVisitRow rw[kChunk];
   const uint32_t nch = *pc.nchunks;
   const uint32_t per = (nch + kXcds - 1) / kXcds;
   const uint32_t local = blockIdx.x / kXcds;
@@ -219,17 +234,24 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
   if (chunk >= nch) return;
   const uint32_t cf = pc.cfirst[chunk], cc = pc.ccount[chunk];
   const int lane = threadIdx.x;
-  const uint32_t slot = (uint32_t)lane / kSub, sub = (uint32_t)lane % kSub;
-  const bool active = slot < cc;
-  const uint32_t i = pc.rows[cf + (active ? slot : 0)];
+  // lanes 0..cc-1 own one row each: its record in LDS, its position in registers
+  const bool owner = (uint32_t)lane < cc;
+  const uint32_t i = pc.rows[cf + (owner ? (uint32_t)lane : 0u)];
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
-  const uint32_t rb_i = pc.refb[i];
   const double hi = pc.h[i];
-  const double lf_i = pc.list_factor * pc.h_list[i];
-  const double supp_i = pc.cutoff2 * hi;
-  const bool tgt_i = need_neigh(pc.bc[i]);
-  const ChunkBox box = chunk_box(pc.tr, xi);
+  if (owner)
+    rw[lane] = AdjVisitRow{xi[0], xi[1], xi[2], hi, pc.list_factor * pc.h_list[i], pc.cutoff2 * hi,
+                           pc.refb[i], (int)need_neigh(pc.bc[i])};
+  // the chunk's grown box (every lane's row counts, as in the passes)
+  const uint32_t slot = (uint32_t)lane / kSub;
+  const uint32_t ib = pc.rows[cf + (slot < cc ? slot : 0u)];
+  const double xb[3] = {pc.x[ib], pc.y[ib], pc.z[ib]};
+  const ChunkBox box = chunk_box(pc.tr, xb);
+  __syncthreads();
   const GridDesc &g = pc.tr.g;
+  const uint32_t cq = 2u * ((uint32_t)lane & 31u) + ((uint32_t)lane >> 5);   // this lane's candidate
+  const uint32_t mine = (uint32_t)lane >> 1;                                 // this lane's row slot
+  const bool odd = (lane & 1) != 0;
   const uint32_t w0 = voff[chunk];
   uint32_t w = w0;
   for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
@@ -240,44 +262,51 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
         uint32_t tl = pc.tr.toff[cell];
         for (uint32_t base = lo; base < hi_; base += kWave, tl++) {
           if (!tile_meets(box, pc.tr.tbox, tl)) continue;
-          const uint32_t j = base + lane;
-          if (j < hi_) {
-            sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
-            sh[lane] = pc.h[j];
-            slf[lane] = pc.list_factor * pc.h_list[j];
-            srefb[lane] = pc.refb[j];
-            sbc[lane] = pc.bc[j];
-          } else {
-            sx[lane] = sy[lane] = sz[lane] = kFar;
-            sh[lane] = slf[lane] = 0;
-            srefb[lane] = ~0u;
-            sbc[lane] = -1;
-          }
-          __syncthreads();
           const uint32_t cnt = min((uint32_t)kWave, hi_ - base);
-          uint32_t mo = 0, mn = 0;
-          if (active) {
-            const uint32_t nb = (cnt + kSub - 1) / kSub;
-#pragma unroll 2
-            for (uint32_t b = 0; b < nb; b++) {
-              const uint32_t q = sub + b * kSub;
-              const double amax = fmax(fabs(xi[0] - sx[q]), fmax(fabs(xi[1] - sy[q]), fabs(xi[2] - sz[q])));
-              const bool same = (pc.list_all != 0) | (srefb[q] == rb_i);
-              const double hj = sh[q], hij = 0.5 * (hi + hj);
-              bool own = tgt_i & (same | (amax <= lf_i)) & (amax <= supp_i);
-              own &= amax <= cutx * fmax(hi, hij);
-              bool nbr = need_neigh(sbc[q]) & (same | (amax <= slf[q])) & (amax <= pc.cutoff2 * hj);
-              nbr &= amax <= cutx * fmax(hj, hij);
-              mo |= (uint32_t)own << b;
-              mn |= (uint32_t)nbr << b;
-            }
+          const bool valid = cq < cnt;
+          const uint32_t j = base + (valid ? cq : 0u);
+          const double cxp = valid ? pc.x[j] : kFar;
+          const double cyp = valid ? pc.y[j] : kFar;
+          const double czp = valid ? pc.z[j] : kFar;
+          const double hj = valid ? pc.h[j] : 0.0;
+          const double lfj = valid ? pc.list_factor * pc.h_list[j] : 0.0;
+          const uint32_t crb = valid ? pc.refb[j] : ~0u;
+          const bool nn_j = valid && need_neigh(pc.bc[j]);
+          const double c2hj = pc.cutoff2 * hj;
+          // rows within reach of the tile in either role
+          const double hmax_t = wave_max(hj);
+          const float4 tlo = pc.tr.tbox[2 * tl], thi = pc.tr.tbox[2 * tl + 1];
+          const double r_i = cutx * fmax(hi, hmax_t);
+          const double dxl = fmax((double)tlo.x - xi[0], xi[0] - (double)thi.x);
+          const double dyl = fmax((double)tlo.y - xi[1], xi[1] - (double)thi.y);
+          const double dzl = fmax((double)tlo.z - xi[2], xi[2] - (double)thi.z);
+          const bool meets = owner && dxl <= r_i && dyl <= r_i && dzl <= r_i;
+          uint64_t tm = __ballot(meets);
+          if (!tm) continue;
+          uint64_t mask = 0;
+          while (tm) {
+            const uint32_t t = (uint32_t)__builtin_ctzll(tm);
+            tm &= tm - 1;
+            const AdjVisitRow R = rw[t];
+            const double amax = fmax(fabs(R.x - cxp), fmax(fabs(R.y - cyp), fabs(R.z - czp)));
+            const bool same = (pc.list_all != 0) | (crb == R.refb);
+            const double hij = 0.5 * (R.h + hj);
+            // row t sums, this lane's candidate is its neighbour
+            const bool own = (R.tgt != 0) & (same | (amax <= R.lf)) & (amax <= R.supp) &
+                             (amax <= cutx * fmax(R.h, hij));
+            // this lane's candidate sums, row t is its neighbour
+            const bool nbr = nn_j & (same | (amax <= lfj)) & (amax <= c2hj) & (amax <= cutx * fmax(hj, hij));
+            const uint64_t bo = __ballot(own), bn = __ballot(nbr);
+            // lanes 2t and 2t+1 (row t's sub-lanes) take the two halves
+            const uint64_t ho = odd ? (bo >> 32) : (bo & 0xffffffffull);
+            const uint64_t hn = odd ? (bn >> 32) : (bn & 0xffffffffull);
+            mask = mine == t ? (ho | hn << 32) : mask;
           }
-          if (__ballot((mo | mn) != 0)) {
+          if (__ballot(mask != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
-            __builtin_nontemporal_store((uint64_t)mo | (uint64_t)mn << 32, vmask + (size_t)w * kWave + lane);
+            __builtin_nontemporal_store(mask, vmask + (size_t)w * kWave + lane);
             w++;
           }
-          __syncthreads();
         }
       }
   if (lane == 0) vnum[chunk] = w - w0;
