@@ -1541,6 +1541,7 @@ __global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double
                              const double *svz, int32_t *bc, double *h, int32_t *inv,
                              double ground, double h_new, double lo0, double lo1, double lo2,
                              double hi0, double hi1, double hi2, unsigned long long *dead, PosTape tp) {
+  // dead[0]: escaped particles, dead[1]: real particles (bc 100/99) after the update
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int b = bc[i];
@@ -1569,10 +1570,14 @@ __global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double
                (x[i] - hi0) > TINY || (y[i] - hi1) > TINY || (z[i] - hi2) > TINY;
     if (tp.mode == 2) esc = tp.esc_step[tp.pid[i]] == tp.k;
     if (esc) {
-      bc[i] = -1; atomicAdd(dead, 1ull);
+      b = -1; bc[i] = -1; atomicAdd(dead, 1ull);
       if (tp.mode == 1) tp.esc_step[tp.pid[i]] = tp.k;
     }
   }
+  // one atomic per wavefront, by its lowest active lane
+  const uint64_t live = __ballot(b == 100 || b == 99);
+  if ((threadIdx.x & (kWave - 1)) == (unsigned)__builtin_ctzll(__ballot(1)) && live)
+    atomicAdd(dead + 1, (unsigned long long)__popcll(live));
 }
 
 __global__ void k_iota_u32(uint32_t *p, uint32_t first, uint32_t n) {
@@ -1717,7 +1722,7 @@ struct psph_ctx {
   double time = 0, next_output = 0;
   int64_t step = 0;
   int ioutput = 0;
-  int64_t last_injected = 0, last_removed = 0;
+  int64_t last_injected = 0, last_removed = 0, last_real = -1;
   unsigned long long last_pairs = 0;
   HostParticles inj;
   // persistent particle ids: load order, then injection order
@@ -2387,6 +2392,7 @@ extern "C" int psph_update_pos(psph_ctx *c, double dt) {
   PosTape tp{c->tape.mode, (int32_t)c->tape.k, c->pid, c->tape.tr_step, c->tape.esc_step};
   if (tp.mode && (!tp.tr_step || c->next_pid > c->tape.id_cap)) return fail(PSPH_EINVAL, "update_pos: tape has no id table");
   StageTimer tm(c, 6);
+  hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 4, 0ull, 1);
   if (n)
     hipLaunchKernelGGL(k_update_pos, blocks(n), 256, 0, c->stream, n, k.dim, k.lans, iferupt(c), dt,
                        c->f.x, c->f.y, c->f.z, c->f.vx, c->f.vy, c->f.vz, c->f.svx, c->f.svy, c->f.svz,
@@ -2395,10 +2401,11 @@ extern "C" int psph_update_pos(psph_ctx *c, double dt) {
                        c->d_u64 + 3, tp);
   tm.stop();
   HIP_TRY(hipGetLastError());
-  unsigned long long dead = 0;
-  HIP_TRY(hipMemcpyAsync(&dead, c->d_u64 + 3, 8, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long cnt[2] = {0, 0};      // escaped, real after the update
+  HIP_TRY(hipMemcpyAsync(cnt, c->d_u64 + 3, 16, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  c->last_removed = (int64_t)dead;
+  c->last_removed = (int64_t)cnt[0];
+  c->last_real = (int64_t)cnt[1];
   c->grid_valid = false;
   return 0;
 }
@@ -2579,7 +2586,7 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
   if (out) {
     out->dt = dt; out->time = c->time; out->step = c->step;
     out->n_total = (int64_t)c->n - c->last_removed;
-    out->n_real = -1;
+    out->n_real = c->last_real;            // the inlet adds erupt ghosts only
     out->n_injected = added; out->n_removed = c->last_removed;
     out->n_pairs = (int64_t)c->last_pairs;
     out->output = output;

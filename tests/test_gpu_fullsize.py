@@ -69,6 +69,7 @@ def test_fullsize_invariants(plume):
     assert inj > 0
     assert len(Q) == len(P) + inj - rem
     assert outs[-1].n_total == len(Q)
+    assert outs[-1].n_real == np.isin(Q["bc"], (100, 99)).sum()
     if rem == 0:
         m_want = math.fsum(P["m"]) + inj * cfg.vent_mass
         assert abs(math.fsum(Q["m"]) - m_want) <= 1e-13 * m_want

@@ -79,7 +79,9 @@ def test_plume_steps(nsteps, adapt):
             assert o.dt > 0
         tg, sg = G.time()
         assert sg == S.st.step and abs(tg - S.st.time) <= 1e-12 * abs(S.st.time)
-        errs = compare(G.read(), S.particles(), FLOAT_FIELDS_3D, REL_TOL_FP64)
+        Q = G.read()
+        assert o.n_real == np.isin(Q["bc"], (100, 99)).sum() and o.n_total == len(Q)
+        errs = compare(Q, S.particles(), FLOAT_FIELDS_3D, REL_TOL_FP64)
         say(f"max rel err {max(errs.values()):.3e}")
         cons_g, cons_o = G.conservation(), S.conservation()
         assert np.allclose(cons_g, cons_o, rtol=1e-10, atol=1e-10 * np.abs(cons_o).max())
