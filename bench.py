@@ -105,9 +105,9 @@ def strong_leg(args):
         f"{w_last:.1f}s ({1e3 * w_last / args.strong_adjoint:.1f} ms/step, {n_real} real of {n_total})")
     free, _ = torch.cuda.mem_get_info()
     # a step record holds 30 doubles and 6 words per particle of capacity; the
-    # adjoint's own arrays, visit lists and checkpoints take most of the rest
+    # adjoint's own arrays, visit lists and checkpoints take much of the rest
     per_step = 264.0 * S.capacity
-    K = int(max(1, min(args.strong_adjoint, 0.3 * free / per_step)))
+    K = int(max(1, min(args.strong_adjoint, 0.45 * free / per_step)))
     log(f"[bench] strong adjoint: {free / 1e9:.0f} GB free, {per_step / 1e9:.1f} GB per step record, "
         f"checkpoint every {K}")
     S.load(Q, t_sim, step)

@@ -1243,10 +1243,15 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
   const int64_t nseg = (nsteps + K - 1) / K;
   if ((int64_t)A.ck.size() < nseg) A.ck.resize(nseg);
   if ((int64_t)A.recs.size() < K) A.recs.resize(K);
-  const int64_t last0 = (nseg - 1) * K;
+  // Segments of K steps, the shorter one (if any) first: the last segment is
+  // reversed straight from the recorded run, the others are recomputed from
+  // their checkpoints, so a full last segment recomputes the fewest steps.
+  const int64_t first_len = nsteps - (nseg - 1) * K;
+  auto seg_begin = [&](int64_t seg) { return seg == 0 ? (int64_t)0 : first_len + (seg - 1) * K; };
+  const int64_t last0 = seg_begin(nseg - 1);
   int rc = 0;
-  for (int64_t s = 0; s < nsteps && !rc; s++) {
-    if (s % K == 0) rc = snap_save(c, A.ck[s / K]);
+  for (int64_t s = 0, seg = 0; s < nsteps && !rc; s++) {
+    if (seg < nseg && s == seg_begin(seg)) rc = snap_save(c, A.ck[seg++]);
     if (rc) break;
     c->rec = s >= last0 ? &A.recs[s - last0] : nullptr;
     rc = psph_step(c, nullptr);
@@ -1282,7 +1287,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
   ThetaGrad th;
   tp.mode = 2;
   for (int64_t seg = nseg - 1; seg >= 0 && !rc; seg--) {
-    const int64_t s0 = seg * K, s1 = std::min(nsteps, s0 + K);
+    const int64_t s0 = seg_begin(seg), s1 = seg + 1 < nseg ? seg_begin(seg + 1) : nsteps;
     if (seg != nseg - 1) {
       rc = snap_load(c, A.ck[seg]);
       for (int64_t s = s0; s < s1 && !rc; s++) {
