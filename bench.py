@@ -108,6 +108,8 @@ def strong_leg(args):
     # adjoint's own arrays, visit lists and checkpoints take much of the rest
     per_step = 264.0 * S.capacity
     K = int(max(1, min(args.strong_adjoint, 0.45 * free / per_step)))
+    if args.strong_ckpt > 0:
+        K = min(K, args.strong_ckpt)
     log(f"[bench] strong adjoint: {free / 1e9:.0f} GB free, {per_step / 1e9:.1f} GB per step record, "
         f"checkpoint every {K}")
     S.load(Q, t_sim, step)
@@ -340,6 +342,8 @@ def main():
     ap.add_argument("--strong-particles", type=float, default=16.0e6)
     ap.add_argument("--strong-steps", type=int, default=100)
     ap.add_argument("--strong-adjoint", type=int, default=50, help="adjoint over the last N steps")
+    ap.add_argument("--strong-ckpt", type=int, default=0,
+                    help="checkpoint interval of the strong adjoint (default: sized to the free HBM)")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="GiB: run only a plain device copy of this size (PMC calibration)")
     args = ap.parse_args()

@@ -1261,6 +1261,9 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
   double H = 0;
   PSPH_TRY(psph_diag_height(c, &H));
   PSPH_TRY(snap_save(c, A.fin));
+  // ids of the whole run (the recompute of a segment restores the id counter
+  // of its checkpoint, so it is read here, after the recorded run)
+  const uint32_t pid_end = c->next_pid;
   HIP_TRY(hipStreamSynchronize(st));
   const double t_fwd = now_ms();
 
@@ -1306,7 +1309,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
   const Snapshot &S0 = A.ck[0];
   std::vector<int32_t> bc0(n0);
   std::vector<double> x0(n0), y0(n0), lam0(6 * std::max<size_t>(n0, 1));
-  std::vector<double> lm(c->next_pid);
+  std::vector<double> lm(pid_end);
   if (n0) {
     HIP_TRY(hipMemcpyAsync(bc0.data(), S0.i[0], n0 * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(x0.data(), S0.d[0], n0 * 8, hipMemcpyDeviceToHost, st));
@@ -1314,7 +1317,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
     const int fl[6] = {L_Z, L_VZ, L_RHO, L_E, L_T, L_CP};
     for (int f = 0; f < 6; f++) HIP_TRY(hipMemcpyAsync(lam0.data() + f * n0, A.L[fl[f]], n0 * 8, hipMemcpyDeviceToHost, st));
   }
-  if (c->next_pid) HIP_TRY(hipMemcpyAsync(lm.data(), A.L[L_M], (size_t)c->next_pid * 8, hipMemcpyDeviceToHost, st));
+  if (pid_end) HIP_TRY(hipMemcpyAsync(lm.data(), A.L[L_M], (size_t)pid_end * 8, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   const VentDeriv vd = vent_deriv(k);
   double lm_erupt = 0;
@@ -1329,7 +1332,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
     ghost_theta(k, vd, vel, 0.0, one, th);
     lm_erupt += lm[q];
   }
-  for (size_t q = n0; q < c->next_pid; q++) lm_erupt += lm[q];
+  for (size_t q = n0; q < pid_end; q++) lm_erupt += lm[q];   // every ghost the inlet added
   th.T += lm_erupt * vd.dm_dT;
   th.ng0 += lm_erupt * vd.dm_dn;
 
