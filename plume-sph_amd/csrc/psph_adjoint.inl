@@ -361,10 +361,11 @@ __global__ __launch_bounds__(kWave) void k_adj_pairs(AdjCommon pc, AdjVisits vl,
         const uint32_t q = sub + b * kSub, jq = base + q;
         const double dx[3] = {xi[0] - sx[q], xi[1] - sy[q], xi[2] - sz[q]};
         const double amax = fmax(fabs(dx[0]), fmax(fabs(dx[1]), fabs(dx[2])));
-        const bool fo = (((m.x >> b) & 1u) != 0) & P::own_ok(t, tile, q, jq, i) &
-                        !pc.cut.outside(amax, P::hker_own(hi, sh[q]));
-        const bool fn = (((m.y >> b) & 1u) != 0) & P::nbr_ok(t, tile, q, jq, i) &
-                        !pc.cut.outside(amax, P::hker_nbr(hi, sh[q]));
+        // all operands evaluated (no short-circuit branches), combined as ints
+        const bool fo = ((int)((m.x >> b) & 1u) & (int)P::own_ok(t, tile, q, jq, i) &
+                         (int)!pc.cut.outside(amax, P::hker_own(hi, sh[q]))) != 0;
+        const bool fn = ((int)((m.y >> b) & 1u) & (int)P::nbr_ok(t, tile, q, jq, i) &
+                         (int)!pc.cut.outside(amax, P::hker_nbr(hi, sh[q]))) != 0;
         if (fo | fn) P::pair(a, t, tile, q, dx, amax, hi, sh[q], fo, fn);
       }
     }
