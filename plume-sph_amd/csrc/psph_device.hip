@@ -1345,7 +1345,8 @@ struct BcondArgs {
   const double *x, *y, *z, *h, *m;
   double *rho, *vx, *vy, *vz, *e, *xi, *p, *c, *T, *gam, *cp;
   const int32_t *bc;
-  const uint32_t *cs;
+  const uint32_t *cs, *toff;
+  const float4 *tbox;        // tile boxes of the step's grid (positions at grid time)
   GridDesc g;
   double reach, zg, del, cutoff2;
   double ext_lo[3];
@@ -1423,7 +1424,17 @@ __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
       mine = cur[2] > c1[2] ? kDone : ((uint32_t)cur[2] * a.g.ny + cur[1]) * a.g.nx + cur[0];
     }
     const uint32_t lo = a.cs[cell], hi = a.cs[cell + 1];
-    for (uint32_t base = lo; base < hi; base += kWave) {
+    uint32_t tl = a.toff[cell];
+    for (uint32_t base = lo; base < hi; base += kWave, tl++) {
+      // A tile is staged only if some lane of this cell's box can reach it:
+      // its particles have moved by at most the displacement that the reach
+      // includes, so the tile box at grid time within reach of the image is
+      // necessary for any of them to be within the kernel box now.
+      const float4 tlo = a.tbox[2 * tl], thi = a.tbox[2 * tl + 1];
+      const bool near_t = here && fmax((double)tlo.x - refc[0], refc[0] - (double)thi.x) <= a.reach &&
+                          fmax((double)tlo.y - refc[1], refc[1] - (double)thi.y) <= a.reach &&
+                          fmax((double)tlo.z - refc[2], refc[2] - (double)thi.z) <= a.reach;
+      if (!__any(near_t)) continue;
       const uint32_t j = base + lane;
       bool ok = false;
       if (j < hi) {
@@ -1440,9 +1451,10 @@ __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
         }
       }
       if (!ok) sbk[0][lane] = -1000;            // never within one bucket of an image
+      const bool any_contr = __any(ok);          // e.g. tiles of wall ghosts only
       __syncthreads();
       const uint32_t cnt = min((uint32_t)kWave, hi - base);
-      if (here) {
+      if (here && any_contr) {
         // phase 1: which staged candidates reach this lane's image
         uint64_t mask = 0;
         for (uint32_t q = 0; q < cnt; q++) {
@@ -2493,7 +2505,7 @@ static int apply_bcond_grid(psph_ctx *c, double disp) {
   a.x = c->f.x; a.y = c->f.y; a.z = c->f.z; a.h = c->f.h; a.m = c->f.m;
   a.rho = c->f.rho; a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.e = c->f.e; a.xi = c->f.xi;
   a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.gam = c->f.gam; a.cp = c->f.cp;
-  a.bc = c->f.bc; a.cs = c->cs; a.g = c->grid;
+  a.bc = c->f.bc; a.cs = c->cs; a.toff = c->toff; a.tbox = c->tbox; a.g = c->grid;
   const double hm = std::max(c->h_max, k.ratio_sml_dx * k.h_air);
   a.reach = k.cutoff * hm * (1 + 1e-9) + disp * (1 + 1e-9);
   a.zg = k.dom_lo[2]; a.del = k.bucket_edge; a.cutoff2 = k.cutoff2;
