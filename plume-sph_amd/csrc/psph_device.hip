@@ -731,19 +731,21 @@ template <class R> __device__ inline R rabs(R x);
 template <> __device__ inline double rabs<double>(double x) { return __builtin_fabs(x); }
 template <> __device__ inline float rabs<float>(float x) { return __builtin_fabsf(x); }
 
-// exp(-e) in FP64 for 0 <= e < kETab / 16 from a table of exp(-k/16) and the
-// Taylor series of exp(-r), 0 <= r < 1/16, to r^8 (truncation < 4e-17).
-constexpr int kETab = 448;            // covers the kernel box: e = |s|^2 <= 27
+// exp(-e) in FP64 for 0 <= e < kETab / 8 from a table of exp(-k/8) and the
+// Taylor series of exp(-r), 0 <= r < 1/8, to r^10 (truncation < 6e-18).  The
+// table lives in each workgroup's LDS; 224 entries (1.75 KB) keep the density
+// pass at 6 waves per SIMD.
+constexpr int kETab = 224;            // covers the kernel box: e = |s|^2 <= 27
 __device__ inline void etab_fill(double *tab, int lane) {
-  for (int k = lane; k < kETab; k += kWave) tab[k] = exp(-0.0625 * k);
+  for (int k = lane; k < kETab; k += kWave) tab[k] = exp(-0.125 * k);
 }
 __device__ inline double exp_neg_tab(double e, const double *tab) {
-  const double k = fmin(floor(e * 16.0), (double)(kETab - 1));
-  const double r = e - 0.0625 * k;
-  double p = 1.0 / 40320;
-  p = fma(p, -r, 1.0 / 5040); p = fma(p, -r, 1.0 / 720); p = fma(p, -r, 1.0 / 120);
-  p = fma(p, -r, 1.0 / 24); p = fma(p, -r, 1.0 / 6); p = fma(p, -r, 0.5);
-  p = fma(p, -r, 1.0); p = fma(p, -r, 1.0);
+  const double k = fmin(floor(e * 8.0), (double)(kETab - 1));
+  const double r = e - 0.125 * k;
+  double p = 1.0 / 3628800;
+  p = fma(p, -r, 1.0 / 362880); p = fma(p, -r, 1.0 / 40320); p = fma(p, -r, 1.0 / 5040);
+  p = fma(p, -r, 1.0 / 720); p = fma(p, -r, 1.0 / 120); p = fma(p, -r, 1.0 / 24);
+  p = fma(p, -r, 1.0 / 6); p = fma(p, -r, 0.5); p = fma(p, -r, 1.0); p = fma(p, -r, 1.0);
   return tab[(int)k] * p;
 }
 
@@ -1553,7 +1555,6 @@ __global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double
                              const double *svz, int32_t *bc, double *h, int32_t *inv,
                              double ground, double h_new, double lo0, double lo1, double lo2,
                              double hi0, double hi1, double hi2, unsigned long long *dead, PosTape tp) {
-  // dead[0]: escaped particles, dead[1]: real particles (bc 100/99) after the update
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int b = bc[i];
@@ -1582,14 +1583,28 @@ __global__ void k_update_pos(uint32_t n, int dim, int lans, int erupting, double
                (x[i] - hi0) > TINY || (y[i] - hi1) > TINY || (z[i] - hi2) > TINY;
     if (tp.mode == 2) esc = tp.esc_step[tp.pid[i]] == tp.k;
     if (esc) {
-      b = -1; bc[i] = -1; atomicAdd(dead, 1ull);
+      bc[i] = -1; atomicAdd(dead, 1ull);
       if (tp.mode == 1) tp.esc_step[tp.pid[i]] = tp.k;
     }
   }
-  // one atomic per wavefront, by its lowest active lane
-  const uint64_t live = __ballot(b == 100 || b == 99);
-  if ((threadIdx.x & (kWave - 1)) == (unsigned)__builtin_ctzll(__ballot(1)) && live)
-    atomicAdd(dead + 1, (unsigned long long)__popcll(live));
+}
+
+// real particles (bc 100/99): grid-stride count, one atomic per block
+__global__ __launch_bounds__(256) void k_count_real(uint32_t n, const int32_t *bc, unsigned long long *out) {
+  __shared__ unsigned int part[256 / kWave];
+  unsigned int cnt = 0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int b = bc[i];
+    cnt += (b == 100 || b == 99) ? 1u : 0u;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) cnt += (unsigned int)__shfl_xor((int)cnt, o);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sum = 0;
+    for (int w = 0; w < 256 / kWave; w++) sum += part[w];
+    if (sum) atomicAdd(out, sum);
+  }
 }
 
 __global__ void k_iota_u32(uint32_t *p, uint32_t first, uint32_t n) {
@@ -2411,6 +2426,7 @@ extern "C" int psph_update_pos(psph_ctx *c, double dt) {
                        c->f.bc, c->f.h, c->f.inv, k.dom_lo[2], k.ratio_sml_dx * k.h_air,
                        k.dom_lo[0], k.dom_lo[1], k.dom_lo[2], k.dom_hi[0], k.dom_hi[1], k.dom_hi[2],
                        c->d_u64 + 3, tp);
+  if (n) hipLaunchKernelGGL(k_count_real, red_blocks(n), 256, 0, c->stream, n, c->f.bc, c->d_u64 + 4);
   tm.stop();
   HIP_TRY(hipGetLastError());
   unsigned long long cnt[2] = {0, 0};      // escaped, real after the update
