@@ -367,7 +367,7 @@ template <int D, class P>
 __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
   __shared__ uint32_t srefb[kWave];
-  __shared__ int sgh[kWave];
+  __shared__ uint8_t sgh[kWave];   // boundary ghost (bc 1, 2), one byte each
   __shared__ typename P::Tile tile;
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so
   // XCD x runs workgroups x, x+8, ...; give it the contiguous chunk range
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
 template <int D, class P>
 __global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
-  __shared__ int sgh[kWave];
+  __shared__ uint8_t sgh[kWave];   // boundary ghost (bc 1, 2), one byte each
   __shared__ typename P::Tile tile;
   const uint32_t nch = *pc.nchunks;
   const uint32_t per = (nch + kXcds - 1) / kXcds;
@@ -989,6 +989,7 @@ struct DensArgs {
 };
 
 struct DensMixTile { double m64[kWave], irho64[kWave], e64[kWave], etab[kETab]; };
+template <class R> struct DensTileR { R m[kWave], rho[kWave], e[kWave]; };
 
 // The FP32-pair form evaluates the density and energy-smoothing weights in
 // FP64 with the exp table: the pressure gradient of the next step amplifies
@@ -997,9 +998,9 @@ template <class R, bool MIXED = std::is_same<R, float>::value>
 struct DensPass {
   using Args = DensArgs;
   static constexpr bool kMixed = MIXED;
-  struct Tile : std::conditional<kMixed, DensMixTile, NoTile>::type {
+  // the mixed form stages only the FP64 copies it reads
+  struct Tile : std::conditional<kMixed, DensMixTile, DensTileR<R>>::type {
     double hh[kWave];       // h / 2
-    R m[kWave], rho[kWave], e[kWave];
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
   __device__ static void init(Tile &s, int lane) {
@@ -1020,8 +1021,9 @@ struct DensPass {
     t.r1 = t.r2 = t.wn = t.en = 0;
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
-    s.hh[l] = 0.5 * a.h[j]; s.m[l] = (R)a.m[j]; s.rho[l] = (R)a.rho[j]; s.e[l] = (R)a.e[j];
+    s.hh[l] = 0.5 * a.h[j];
     if constexpr (kMixed) { s.m64[l] = a.m[j]; s.irho64[l] = 1.0 / a.rho[j]; s.e64[l] = a.e[j]; }
+    else { s.m[l] = (R)a.m[j]; s.rho[l] = (R)a.rho[j]; s.e[l] = (R)a.e[j]; }
     int ph = a.phase[j];
     s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
   }
@@ -1054,8 +1056,7 @@ struct DensPass {
         t.wn += vr;
         t.en += vr * s.e64[q];
       }
-      return;
-    }
+    } else {
     const R hij = (R)hijd;
     const R ih = rrcp<R>(hij);
     const R mj = s.m[q];
@@ -1083,6 +1084,7 @@ struct DensPass {
         t.wn += (double)vr;
         t.en += (double)(vr * s.e[q]);
       }
+    }
     }
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
