@@ -5,7 +5,7 @@ import numpy as np
 
 import psph
 
-# Parity bars: neighbour sets and ordering bit-exact;
+# Parity bars of the FP64 form: neighbour sets and ordering bit-exact;
 # density, velocity, energy (and the EOS outputs) within 1e-10 relative.
 REL_TOL_FP64 = 1e-10
 FLOAT_FIELDS_3D = ["x", "y", "z", "vx", "vy", "vz", "svx", "svy", "svz", "rho", "e", "m", "h",

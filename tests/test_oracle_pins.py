@@ -78,7 +78,9 @@ def test_eos_pure_air():
 
 
 def test_eos_vent_state_round_trip():
-    """Weak vent: p = p_v exactly after the EOS; the numbers in the survey."""
+    """Weak vent (p_v = 101325 Pa, T = 1053 K, gas fraction 0.03): the EOS
+    returns p_v and T exactly, and rho_v = p_v / (Rg T) / ng0, e_v, gamma_v and
+    the sound speed take their closed-form values (parameters.h:958-975)."""
     c = small_plume_config()
     p, snd, T, gam, cp = oracle.eos(c, c.vent_rho, c.vent_e, 1.0)
     assert p == pytest.approx(101325.0, rel=1e-14)
