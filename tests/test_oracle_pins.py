@@ -149,14 +149,10 @@ def test_product_keys_match_oracle():
 
 
 # ---------------------------------------------------------------- ordering
-def test_slot_formula_and_canonical_order():
-    rng = np.random.default_rng(5)
-    for _ in range(1000):
-        k0, k1 = int(rng.integers(0, 1 << 30)), int(rng.integers(0, 1 << 32))
-        S02 = 4000000 // 10
-        S01 = 4000000 - S02
-        want = int(k0 / 4294967295.0 * S01) + int(k1 / 4294967295.0 * S02)
-        assert oracle.slot(k0, k1, 4000000) == want
+def test_canonical_order():
+    """The slot formula itself is pinned by its special values in
+    test_oracle_closed_forms.py; here the state order is ascending slot, ties
+    broken by (key0, key1, key2); 1D hashes by key0 alone."""
     assert oracle.slot(17, 3, 10000, dim=1) == 17
     cfg, met, P = small_plume()
     S = oracle.State(cfg, P)
