@@ -54,10 +54,16 @@ extern "C" {
                                 filter weights and the h update stay FP64):
                                 faster, but velocities drift ~1e-4 rel L1 from
                                 FP64 over the same run                         */
+#define PSPH_FP32_EXP    3   /* PSPH_FP32_PAIR with the pair weights'
+                                exponential in FP32 hardware (argument formed
+                                in FP64, its rounding taken back out; h, 1/h,
+                                norms and sums FP64): ~1 FP32 ulp per weight  */
 
-/* ---- smoothing kernels ------------------------------------------------- */
-#define PSPH_KERNEL_GAUSS_BOX   0  /* sph_lib.cc:207-253 (the reference's)    */
-#define PSPH_KERNEL_CUBIC_M4    1  /* Monaghan 1992 M4 spline, support 2h     */
+/* ---- smoothing kernel -------------------------------------------------- */
+/* The reference has one kernel, a Gaussian truncated per axis at |s_k| = 3
+ * (USE_GAUSSIAN, sph_lib.cc:207-253, options.h:18-20,454-456); it has no
+ * cubic spline.  psph_create rejects any other value of cfg.kernel. */
+#define PSPH_KERNEL_GAUSS_BOX   0
 
 /*
  * Solver configuration: every compile-time switch (options.h), constant
@@ -82,7 +88,10 @@ typedef struct psph_config {
   int32_t wall_bc;          /* 1: refresh the wall ghosts next to the ground
                                from their mirror images after every step
                                (apply_bcond, main.cc:468); 0: keep their state */
-  int32_t reserved_i[1];
+  int32_t adapt_domain;     /* ADJUST_DOMAIN (main.cc:256-304): 1 grows the
+                               region of real particles bucket by bucket as
+                               involved material reaches its outer layer,
+                               starting from idom (psph_domain_init)          */
 
   double cutoff;            /* CUTOFF = 3: kernel box half-width in h         */
   double cutoff2;           /* CUTOFF2: pair pre-filter box in h_i (8 / 12)   */
@@ -115,7 +124,10 @@ typedef struct psph_config {
   double vent_r, vent_v, vent_T, vent_p, vent_x, vent_y, vent_z;
   double vent_rho, vent_e, vent_gamma, vent_c;
   double vent_sml, vent_mass, vent_bot;
-  double reserved_d[8];
+  double idom_lo[3], idom_hi[3];  /* initial domain (deck lines 10-15,
+                               dataread.cc:326-332); dom is then the largest
+                               one (Ll_P, Lu_P of the grid preprocessor)      */
+  double reserved_d[2];
 } psph_config;
 
 /*
@@ -147,6 +159,8 @@ typedef struct psph_step_out {
   int64_t output;           /* 1: the step reached an output time, the point
                                where the reference's loop writes its files
                                (TimeProps::ifoutput, main.cc:473-477)         */
+  int64_t n_domain;         /* particles the domain adaptation added this step
+                               (pressure and wall ghosts of the new layer)    */
 } psph_step_out;
 
 typedef struct psph_ctx psph_ctx;
@@ -214,6 +228,19 @@ int psph_count(psph_ctx *ctx, size_t *n_total, size_t *n_real);
 int psph_time(psph_ctx *ctx, double *time, int64_t *step);
 /* The configuration the context runs (psph_create's, as completed there). */
 int psph_get_config(psph_ctx *ctx, psph_config *cfg);
+/* Zero-copy view of the context's particle arrays: every pointer of *dev
+ * receives the DEVICE address of that field's array, *n the particle count,
+ * *stream (if not NULL) the context's hipStream_t.  Entries [0, n) are the
+ * particles in the context's current order: the order given to psph_load right
+ * after it, the cell order after a neighbour search.  The call waits for the
+ * context's pending work.  The addresses stay valid until the next call that
+ * loads, sorts (psph_neighbors, psph_step, psph_run) or runs the adjoint: a
+ * sort gathers every field into a spare array and swaps.  Writes through the
+ * view change the state the next stage reads.  The reference hands its
+ * particles out as Particle pointers through THashTable iteration
+ * (thashtab.h THTIterator); this is the device-resident counterpart, used by
+ * the torch wrapper. */
+int psph_device_view(psph_ctx *ctx, psph_soa *dev, size_t *n, void **stream);
 
 /* ---- stages (reference granularity) ------------------------------------ */
 /* dt = CFL * min h/c over need_neigh particles (timestep.cc:94-135, 1D
@@ -244,6 +271,23 @@ int psph_inject(psph_ctx *ctx, double dt, int64_t *added);
  * after psph_load for the refresh the reference makes before its loop
  * (main.cc:196). */
 int psph_apply_bcond(psph_ctx *ctx);
+/* Domain adaptation (ADJUST_DOMAIN; scan_outside_layer.cc:44-121,
+ * adapt_domain.cc:41-194, add_pressure_ghost.cc:37-316, add_wall_ghost.cc:
+ * 45-293, called from main.cc:256-304 and 433-436).  With cfg.adapt_domain,
+ * call this once after psph_load: it sets up the buckets of the grid
+ * preprocessor (edge cfg.bucket_edge over the extended domain), whose
+ * has_involved starts from the initial domain cfg.idom_lo/idom_hi and the
+ * buckets that hold real particles, and keeps the atmosphere, seed and jitter
+ * of the air lattice the new ghost layers are cut from (psph_gen_plume's).
+ * psph_step then ends with scan_outside_layer and, when involved material
+ * reached the outer layer of the region of real particles, starts the next
+ * step by growing that region by the bordering buckets: their particles turn
+ * real (potentially involved), the empty buckets around get pressure ghosts,
+ * the ground layer under them wall ghosts, and the wall ghosts are refreshed
+ * when cfg.wall_bc is set.  psph_gen_plume with cfg.adapt_domain makes the
+ * matching initial state: real air in the initial domain's buckets and the
+ * ghost layers around them. */
+int psph_domain_init(psph_ctx *ctx, const psph_meteo *met, uint32_t seed, double jitter);
 /* One whole step in the order of main.cc:217-486 (3D) / 676-751 (1D). */
 int psph_step(psph_ctx *ctx, psph_step_out *out);
 int psph_run(psph_ctx *ctx, int64_t nsteps, psph_step_out *last);
@@ -270,7 +314,11 @@ int psph_diag_height(psph_ctx *ctx, double *H);
  * erupted solid mass fraction) and every particle's initial velocity, with
  * the step's discrete schedule frozen at the recorded run: time steps,
  * adapted smoothing lengths, inlet spawn counts, erupt-ghost -> real and
- * escape events.  Erupt ghosts present at the start (bc 0) are taken to hold
+ * escape events, and the pair set of every pass (which pairs lie inside the
+ * kernel boxes, the list and support boxes; a box indicator has zero
+ * derivative wherever it is differentiable).  The recorded run keeps every
+ * step's particle positions by id (24 B per particle and step) for
+ * psph_replay_run.  Erupt ghosts present at the start (bc 0) are taken to hold
  * the vent state of the config, as psph_gen_plume makes them.
  * The forward run stores a checkpoint every ckpt_every steps; the reverse
  * sweep recomputes one segment at a time from its checkpoint.  grad_v and
@@ -292,9 +340,13 @@ typedef struct psph_adjoint_out {
 int psph_adjoint_run(psph_ctx *ctx, int64_t nsteps, int64_t ckpt_every, psph_adjoint_out *out,
                      double *grad_v, double *grad_sv, size_t n_cap);
 /* Runs nsteps steps with the frozen schedule of the last psph_adjoint_run
- * (the map the adjoint differentiates; finite-difference checks use it).
- * The context must hold a state loaded at that run's start time and step,
- * with the same particle count. */
+ * (the map the adjoint differentiates; finite-difference checks use it):
+ * besides the schedule, the cell order, tiles, visit lists and every pair
+ * predicate of a step are evaluated on the recorded run's positions of that
+ * step, and only the pair arithmetic on the replay's own, so the replay
+ * visits the recorded pairs in the recorded summation order.  The context
+ * must hold a state loaded at that run's start time and step, with the same
+ * particle count. */
 int psph_replay_run(psph_ctx *ctx, int64_t nsteps);
 /* Sets the vent exit velocity, temperature and gas mass fraction of the
  * context's config and recomputes the derived vent state (psph_config_vent). */

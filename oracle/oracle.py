@@ -38,7 +38,10 @@ class OrcState(C.Structure):
                    ("time", C.c_double), ("next_output", C.c_double), ("step", C.c_int64),
                    ("ioutput", C.c_int),
                    ("last_injected", C.c_int64), ("last_removed", C.c_int64),
-                   ("last_pairs", C.c_int64)])
+                   ("last_pairs", C.c_int64),
+                   ("bval", C.POINTER(C.c_uint8)), ("berupt", C.POINTER(C.c_uint8)),
+                   ("nbk", C.c_int32 * 3), ("adapt_pending", C.c_int32), ("lat_seed", C.c_uint32),
+                   ("lat_jitter", C.c_double), ("met", OrcMeteo), ("last_adapted", C.c_int64)])
 
 
 _lib = None
@@ -71,6 +74,11 @@ def lib():
         "orc_state_store": (C.c_int, [S, C.POINTER(SoA)]),
         "orc_sort_canonical": (None, [CFG, S]),
         "orc_air_lattice": (C.c_int, [CFG, C.POINTER(OrcMeteo), C.c_uint32, C.c_double, S]),
+        "orc_domain_init": (C.c_int, [CFG, C.POINTER(OrcMeteo), C.c_uint32, C.c_double, S]),
+        "orc_domain_setup": (C.c_int, [CFG, C.POINTER(OrcMeteo), C.c_uint32, C.c_double, S]),
+        "orc_domain_scan": (C.c_int, [CFG, S]),
+        "orc_domain_adapt": (C.c_int, [CFG, S]),
+        "orc_bucket_type": (None, [CFG, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "orc_setup_erupt": (C.c_int, [CFG, S]),
         "orc_shock_tube": (C.c_int, [CFG] + [C.c_double] * 7 + [C.c_int] * 4 + [S]),
         "orc_timestep": (C.c_double, [CFG, S]),
@@ -213,6 +221,38 @@ def air_lattice(cfg, met, seed, jitter):
     _check(lib().orc_setup_erupt(C.byref(S.cfg), C.byref(S.st)))
     lib().orc_meteo_free(C.byref(m))
     return S
+
+
+def domain_setup(cfg, met, seed, jitter):
+    """Oracle restatement of the initial state of a run with domain adaptation
+    (the initial domain's buckets, their pressure-ghost and wall-ghost layers,
+    the vent) and its bucket flags."""
+    S = State(cfg)
+    m = meteo(met)
+    _check(lib().orc_domain_setup(C.byref(S.cfg), C.byref(m), seed, jitter, C.byref(S.st)))
+    lib().orc_meteo_free(C.byref(m))
+    return S
+
+
+def domain_init(S, met, seed, jitter):
+    """Bucket flags of an oracle state for domain adaptation."""
+    m = meteo(met)
+    _check(lib().orc_domain_init(C.byref(S.cfg), C.byref(m), seed, jitter, C.byref(S.st)))
+    lib().orc_meteo_free(C.byref(m))
+
+
+def bucket_values(S):
+    """has_involved of every bucket, shape (nbz, nby, nbx)."""
+    nb = tuple(S.st.nbk)
+    n = nb[0] * nb[1] * nb[2]
+    return np.ctypeslib.as_array(S.st.bval, (n,)).copy().reshape(nb[2], nb[1], nb[0])
+
+
+def bucket_type(cfg, bx, by, bz):
+    bt = (C.c_int * 6)()
+    t = C.c_int()
+    lib().orc_bucket_type(C.byref(cfg), bx, by, bz, bt, C.byref(t))
+    return t.value, tuple(bt)
 
 
 def shock_tube(cfg, left, right, mid=0.0, n_nosave_l=40, n_nosave_r=20, n_ghost=35, equal_mass=True):

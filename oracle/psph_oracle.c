@@ -256,6 +256,8 @@ void orc_state_free(orc_state *st) {
 #undef Y
 #undef Z
   free(st->nb_off); free(st->nb_idx);
+  free(st->bval); free(st->berupt);
+  orc_meteo_free(&st->met);
   memset(st, 0, sizeof *st);
 }
 
@@ -1019,9 +1021,30 @@ static void orc_air_state(const psph_config *cfg, const orc_meteo *met, orc_stat
   orc_eos(cfg, st->rho[i], st->e[i], st->xi[i], &st->p[i], &st->c[i], &st->T[i], &st->gam[i], &st->cp[i]);
 }
 
+/* Lattice point l of the air (spacing h_air, offset h_air/2 inside each
+ * bucket): pc0 exact, pc with the seeded jitter of the synthetic inputs. */
+static void orc_lattice_point(const psph_config *cfg, uint32_t seed, double jitter, const int64_t l[3],
+                              const int64_t nl[3], double pc0[3], double pc[3]) {
+  const double dx = cfg->h_air, dx2 = 0.5 * dx, del = cfg->bucket_edge;
+  const int per = (int)llround(del / dx);
+  for (int k = 0; k < 3; k++) {
+    double mincrd = cfg->ext_lo[k] + (double)(l[k] / per) * del;   /* preprocess.cc:147-159 */
+    pc0[k] = pc[k] = mincrd + dx2 + (double)(l[k] % per) * dx;      /* add_air.cc:575-577 */
+  }
+  if (jitter > 0) {
+    uint64_t L = (uint64_t)((l[2] * nl[1] + l[1]) * nl[0] + l[0]);
+    uint32_t ctr[4] = {(uint32_t)L, (uint32_t)(L >> 32), 0, 0}, key[2] = {seed, 0x50535048u}, r[4];
+    orc_philox4x32_10(ctr, key, r);
+    for (int k = 0; k < 3; k++) {
+      double u = ((double)r[k] + 0.5) * (1.0 / 4294967296.0);
+      pc[k] += (2.0 * u - 1.0) * jitter * dx;
+    }
+  }
+}
+
 int orc_air_lattice(const psph_config *cfg, const orc_meteo *met, uint32_t seed,
                     double jitter, orc_state *st) {
-  double dx = cfg->h_air, dx2 = 0.5 * dx, del = cfg->bucket_edge;
+  double dx = cfg->h_air, del = cfg->bucket_edge;
   int nb[3], per = (int)llround(del / dx);
   for (int k = 0; k < 3; k++) nb[k] = (int)llround((cfg->ext_hi[k] - cfg->ext_lo[k]) / del);
   int64_t nl[3] = {(int64_t)nb[0] * per, (int64_t)nb[1] * per, (int64_t)nb[2] * per};
@@ -1031,20 +1054,8 @@ int orc_air_lattice(const psph_config *cfg, const orc_meteo *met, uint32_t seed,
     for (int64_t ly = 0; ly < nl[1]; ly++)
       for (int64_t lx = 0; lx < nl[0]; lx++) {
         int64_t l[3] = {lx, ly, lz};
-        double pc[3];
-        for (int k = 0; k < 3; k++) {
-          double mincrd = cfg->ext_lo[k] + (double)(l[k] / per) * del;   /* preprocess.cc:147-159 */
-          pc[k] = mincrd + dx2 + (double)(l[k] % per) * dx;               /* add_air.cc:575-577 */
-        }
-        if (jitter > 0) {
-          uint64_t L = (uint64_t)((lz * nl[1] + ly) * nl[0] + lx);
-          uint32_t ctr[4] = {(uint32_t)L, (uint32_t)(L >> 32), 0, 0}, key[2] = {seed, 0x50535048u}, r[4];
-          orc_philox4x32_10(ctr, key, r);
-          for (int k = 0; k < 3; k++) {
-            double u = ((double)r[k] + 0.5) * (1.0 / 4294967296.0);
-            pc[k] += (2.0 * u - 1.0) * jitter * dx;
-          }
-        }
+        double pc[3], pc0[3];
+        orc_lattice_point(cfg, seed, jitter, l, nl, pc0, pc);
         int bc;
         if (!(pc[2] > cfg->dom_lo[2])) bc = 2;
         else if (pc[0] < cfg->dom_lo[0] || pc[0] > cfg->dom_hi[0] || pc[1] < cfg->dom_lo[1] ||
@@ -1253,11 +1264,358 @@ int orc_apply_bcond(const psph_config *cfg, orc_state *st) {
   return 0;
 }
 
+/* ===================================================================== */
+/* Domain adaptation (ADJUST_DOMAIN, main.cc:256-304 and 433-436).         */
+/* Buckets are the preprocessor's cells of edge del = PARTICLE_DENSITY      */
+/* h_air over the extended domain; a bucket's type comes from its faces     */
+/* against the largest domain dom (preprocess.cc:164-199, 380-440).  Each   */
+/* bucket carries has_involved (bit 0 potentially involved, bit 1 involved, */
+/* bucket.h:400-414, 579-589) and erupt_flag.  At the end of a step          */
+/* scan_outside_layer marks the buckets of value 1 that hold an involved     */
+/* particle with both bits and counts those next to a non-empty, uninvolved */
+/* bucket that is not underground; the next step, after its time step,      */
+/* adapt_domain turns the particles of the uninvolved buckets next to an    */
+/* involved one real (potentially involved), add_pressure_ghost fills the   */
+/* empty uninvolved buckets next to any involved or potentially involved one */
+/* with pressure ghosts, add_wall_ghost the ground layer and the layer below */
+/* with wall ghosts, and the wall ghosts are refreshed.  Bucket bookkeeping */
+/* of the reference's hash tables (brief buckets, active marks, guests) has */
+/* no effect on the particles and is not restated.                          */
+/* ===================================================================== */
+enum { ORC_UNDERGROUND = 1, ORC_MIXED = 2, ORC_OVERGROUND = 3, ORC_PRESS_BC = 4 };
+
+static int orc_face(double crd, double max, double min) {    /* preprocess.cc:386-399 */
+  if (crd < min) return -1;
+  if (crd > max) return 1;
+  return 0;
+}
+
+static void orc_bucket_box(const psph_config *cfg, int b, int k, double *lo, double *hi) {
+  *lo = cfg->ext_lo[k] + b * cfg->bucket_edge;                /* preprocess.cc:166-179 */
+  *hi = cfg->ext_lo[k] + (b + 1) * cfg->bucket_edge;
+}
+
+void orc_bucket_type(const psph_config *cfg, int bx, int by, int bz, int bt[6], int *type) {
+  const int b[3] = {bx, by, bz};
+  int flag[3], sum = 0;
+  for (int k = 0; k < 3; k++) {
+    double lo, hi;
+    orc_bucket_box(cfg, b[k], k, &lo, &hi);
+    bt[2 * k] = orc_face(lo, cfg->dom_hi[k], cfg->dom_lo[k]);
+    bt[2 * k + 1] = orc_face(hi, cfg->dom_hi[k], cfg->dom_lo[k]);
+    flag[k] = abs(bt[2 * k] + bt[2 * k + 1]);
+    sum += flag[k];
+  }
+  if (flag[0] == 2 || flag[1] == 2 || flag[2] == 2)          /* preprocess.cc:419-433 */
+    *type = (bt[4] == -1 && bt[5] == -1) ? ORC_UNDERGROUND : ORC_PRESS_BC;
+  else if (flag[0] == 1 || flag[1] == 1 || flag[2] == 1)
+    *type = ORC_MIXED;
+  else
+    *type = sum == 0 ? ORC_OVERGROUND : 0;
+}
+
+/* a box [lo, hi] meets the bucket unless it lies wholly on one side of it
+ * on some axis (determine_bucket_involved_flag, dataread.cc:52-75, and
+ * determine_erupt_buket, sph_lib.cc:1597-1620) */
+static int orc_box_meets_bucket(const psph_config *cfg, const int b[3], const double *lo, const double *hi,
+                                int box_vs_bucket) {
+  for (int k = 0; k < 3; k++) {
+    double blo, bhi;
+    orc_bucket_box(cfg, b[k], k, &blo, &bhi);
+    int f0, f1;
+    if (box_vs_bucket) { f0 = orc_face(lo[k], bhi, blo); f1 = orc_face(hi[k], bhi, blo); }
+    else { f0 = orc_face(blo, hi[k], lo[k]); f1 = orc_face(bhi, hi[k], lo[k]); }
+    if (abs(f0 + f1) == 2) return 0;
+  }
+  return 1;
+}
+
+static size_t orc_nbuckets(const orc_state *st) { return (size_t)st->nbk[0] * st->nbk[1] * st->nbk[2]; }
+static size_t orc_bidx(const orc_state *st, int bx, int by, int bz) {
+  return ((size_t)bz * st->nbk[1] + by) * st->nbk[0] + bx;
+}
+
+int orc_domain_init(const psph_config *cfg, const orc_meteo *met, uint32_t seed, double jitter,
+                    orc_state *st) {
+  for (int k = 0; k < 3; k++) st->nbk[k] = (int)llround((cfg->ext_hi[k] - cfg->ext_lo[k]) / cfg->bucket_edge);
+  const size_t nb = orc_nbuckets(st);
+  free(st->bval); free(st->berupt);
+  st->bval = calloc(nb ? nb : 1, 1);
+  st->berupt = calloc(nb ? nb : 1, 1);
+  if (!st->bval || !st->berupt) ORC_FAIL(PSPH_ENOMEM, "oom (bucket flags)");
+  orc_meteo_free(&st->met);
+  orc_meteo_from_arrays(&st->met, met->rows, met->nprop, met->z, met->data);
+  st->lat_seed = seed;
+  st->lat_jitter = jitter;
+  st->adapt_pending = 0;
+  /* the vent duct (setup_erupt.cc:86-91): the vent disc over one extended-  */
+  /* domain bottom layer below the ground                                    */
+  const double dlo[3] = {cfg->vent_x - cfg->vent_r, cfg->vent_y - cfg->vent_r, cfg->vent_z - 1.5 * cfg->bucket_edge};
+  const double dhi[3] = {cfg->vent_x + cfg->vent_r, cfg->vent_y + cfg->vent_r, cfg->vent_z};
+  for (int bz = 0; bz < st->nbk[2]; bz++)
+    for (int by = 0; by < st->nbk[1]; by++)
+      for (int bx = 0; bx < st->nbk[0]; bx++) {
+        const int b[3] = {bx, by, bz};
+        const size_t q = orc_bidx(st, bx, by, bz);
+        /* potentially involved: the bucket meets the initial domain */
+        st->bval[q] = (uint8_t)orc_box_meets_bucket(cfg, b, cfg->idom_lo, cfg->idom_hi, 0);
+        st->berupt[q] = (uint8_t)orc_box_meets_bucket(cfg, b, dlo, dhi, 1);
+      }
+  /* a state loaded mid-run: buckets holding real particles are potentially */
+  /* involved as well                                                        */
+  for (size_t i = 0; i < st->n; i++) {
+    int b[3];
+    if (is_real(st->bc[i]) && orc_bucket_of(cfg, st, i, b) == 0) st->bval[orc_bidx(st, b[0], b[1], b[2])] |= 1;
+  }
+  return 0;
+}
+
+/* per-bucket occupancy: bit 0 any particle, bit 1 an involved one (2),     */
+/* bit 2 a pressure ghost, bit 3 a wall ghost                               */
+static uint8_t *orc_bucket_occupancy(const psph_config *cfg, const orc_state *st) {
+  uint8_t *occ = calloc(orc_nbuckets(st) ? orc_nbuckets(st) : 1, 1);
+  if (!occ) return NULL;
+  for (size_t i = 0; i < st->n; i++) {
+    int b[3];
+    if (orc_bucket_of(cfg, st, i, b)) continue;
+    uint8_t f = 1;
+    if (st->inv[i] == 2) f |= 2;
+    if (st->bc[i] == 1) f |= 4;
+    if (st->bc[i] == 2) f |= 8;
+    occ[orc_bidx(st, b[0], b[1], b[2])] |= f;
+  }
+  return occ;
+}
+
+static int orc_any_neighbour(const orc_state *st, int bx, int by, int bz, uint8_t mask) {
+  for (int dz = -1; dz <= 1; dz++)
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        if (!dx && !dy && !dz) continue;
+        const int x = bx + dx, y = by + dy, z = bz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= st->nbk[0] || y >= st->nbk[1] || z >= st->nbk[2]) continue;
+        if (st->bval[orc_bidx(st, x, y, z)] & mask) return 1;
+      }
+  return 0;
+}
+
+int orc_domain_scan(const psph_config *cfg, orc_state *st) {      /* scan_outside_layer.cc:44-121 */
+  if (!st->bval) ORC_FAIL(PSPH_EINVAL, "domain adaptation: call orc_domain_init first");
+  uint8_t *occ = orc_bucket_occupancy(cfg, st);
+  if (!occ) ORC_FAIL(PSPH_ENOMEM, "oom (bucket occupancy)");
+  int adapt = 0;
+  for (int bz = 0; bz < st->nbk[2]; bz++)
+    for (int by = 0; by < st->nbk[1]; by++)
+      for (int bx = 0; bx < st->nbk[0]; bx++) {
+        const size_t q = orc_bidx(st, bx, by, bz);
+        if (st->bval[q] != 1 || !(occ[q] & 2)) continue;
+        st->bval[q] = 3;                                          /* HAS_BOTH */
+        for (int dz = -1, done = 0; dz <= 1 && !done; dz++)
+          for (int dy = -1; dy <= 1 && !done; dy++)
+            for (int dx = -1; dx <= 1 && !done; dx++) {
+              if (!dx && !dy && !dz) continue;
+              const int x = bx + dx, y = by + dy, z = bz + dz;
+              if (x < 0 || y < 0 || z < 0 || x >= st->nbk[0] || y >= st->nbk[1] || z >= st->nbk[2]) continue;
+              const size_t r = orc_bidx(st, x, y, z);
+              int bt[6], type;
+              orc_bucket_type(cfg, x, y, z, bt, &type);
+              if ((occ[r] & 1) && st->bval[r] == 0 && type != ORC_UNDERGROUND) { adapt++; done = 1; }
+            }
+      }
+  free(occ);
+  st->adapt_pending = adapt;
+  return 0;
+}
+
+/* a new air particle at lattice point l (add_pressure_ghost.cc:164-232,    */
+/* add_wall_ghost.cc:150-205): initial_air state, keys with this step       */
+static int orc_add_lattice_particle(const psph_config *cfg, orc_state *st, const double pc[3], int bc) {
+  if (orc_state_reserve(st, st->n + 1 > st->cap ? 2 * st->cap + 1024 : st->cap)) return PSPH_ENOMEM;
+  size_t i = orc_push(st);
+  for (int k = 0; k < 3; k++) st->x[k][i] = pc[k];
+  double normc[3];
+  for (int k = 0; k < 3; k++) normc[k] = (pc[k] - cfg->ext_lo[k]) / (cfg->ext_hi[k] - cfg->ext_lo[k]);
+  uint32_t key[3];
+  orc_thsfc3d(normc, (uint32_t)st->step, key);
+  for (int k = 0; k < 3; k++) st->key[k][i] = key[k];
+  st->h[i] = cfg->h_air;
+  st->bc[i] = bc;
+  st->phase[i] = 1;
+  st->inv[i] = 0;
+  st->xi[i] = 0.;
+  st->cp[i] = 0.;
+  orc_air_state(cfg, &st->met, st, i);
+  return 0;
+}
+
+/* calls fn on the lattice points of bucket (bx, by, bz) */
+typedef int (*orc_lat_fn)(const psph_config *, orc_state *, const double pc0[3], const double pc[3], void *);
+static int orc_bucket_lattice(const psph_config *cfg, orc_state *st, int bx, int by, int bz, orc_lat_fn fn, void *ud) {
+  const int per = (int)llround(cfg->bucket_edge / cfg->h_air);
+  const int64_t nl[3] = {(int64_t)st->nbk[0] * per, (int64_t)st->nbk[1] * per, (int64_t)st->nbk[2] * per};
+  const int b[3] = {bx, by, bz};
+  for (int i = 0; i < per; i++)
+    for (int j = 0; j < per; j++)
+      for (int k = 0; k < per; k++) {
+        const int64_t l[3] = {(int64_t)b[0] * per + i, (int64_t)b[1] * per + j, (int64_t)b[2] * per + k};
+        double pc0[3], pc[3];
+        orc_lattice_point(cfg, st->lat_seed, st->lat_jitter, l, nl, pc0, pc);
+        int rc = fn(cfg, st, pc0, pc, ud);
+        if (rc) return rc;
+      }
+  return 0;
+}
+
+static int orc_lat_pghost_above(const psph_config *cfg, orc_state *st, const double pc0[3], const double pc[3], void *ud) {
+  (void)ud;
+  return pc0[2] > cfg->dom_lo[2] ? orc_add_lattice_particle(cfg, st, pc, 1) : 0;
+}
+static int orc_lat_pghost(const psph_config *cfg, orc_state *st, const double pc0[3], const double pc[3], void *ud) {
+  (void)ud; (void)pc0;
+  return orc_add_lattice_particle(cfg, st, pc, 1);
+}
+static int orc_lat_wghost_below(const psph_config *cfg, orc_state *st, const double pc0[3], const double pc[3], void *ud) {
+  (void)ud;
+  return pc0[2] < cfg->dom_lo[2] ? orc_add_lattice_particle(cfg, st, pc, 2) : 0;
+}
+static int orc_lat_wghost(const psph_config *cfg, orc_state *st, const double pc0[3], const double pc[3], void *ud) {
+  (void)ud; (void)pc0;
+  return orc_add_lattice_particle(cfg, st, pc, 2);
+}
+static int orc_lat_real_above(const psph_config *cfg, orc_state *st, const double pc0[3], const double pc[3], void *ud) {
+  (void)ud;
+  if (!(pc0[2] > cfg->dom_lo[2])) return 0;
+  int rc = orc_add_lattice_particle(cfg, st, pc, 100);
+  st->inv[st->n - 1] = 1;
+  return rc;
+}
+static int orc_lat_real(const psph_config *cfg, orc_state *st, const double pc0[3], const double pc[3], void *ud) {
+  (void)ud; (void)pc0;
+  int rc = orc_add_lattice_particle(cfg, st, pc, 100);
+  st->inv[st->n - 1] = 1;
+  return rc;
+}
+
+static int orc_add_pressure_ghost(const psph_config *cfg, orc_state *st) {  /* add_pressure_ghost.cc:37-316 */
+  uint8_t *occ = orc_bucket_occupancy(cfg, st);
+  if (!occ) ORC_FAIL(PSPH_ENOMEM, "oom (bucket occupancy)");
+  int rc = 0;
+  for (int bz = 0; bz < st->nbk[2] && !rc; bz++)
+    for (int by = 0; by < st->nbk[1] && !rc; by++)
+      for (int bx = 0; bx < st->nbk[0] && !rc; bx++) {
+        const size_t q = orc_bidx(st, bx, by, bz);
+        if (st->bval[q] || (occ[q] & 4) || st->berupt[q]) continue;
+        if (!orc_any_neighbour(st, bx, by, bz, 3)) continue;
+        int bt[6], type;
+        orc_bucket_type(cfg, bx, by, bz, bt, &type);
+        if (type == ORC_MIXED)
+          rc = orc_bucket_lattice(cfg, st, bx, by, bz, bt[4] == -1 ? orc_lat_pghost_above : orc_lat_pghost, NULL);
+        else if (type == ORC_OVERGROUND || type == ORC_PRESS_BC)
+          rc = orc_bucket_lattice(cfg, st, bx, by, bz, orc_lat_pghost, NULL);
+      }
+  free(occ);
+  return rc;
+}
+
+static int orc_add_wall_ghost(const psph_config *cfg, orc_state *st) {      /* add_wall_ghost.cc:45-293 */
+  uint8_t *occ = orc_bucket_occupancy(cfg, st);
+  if (!occ) ORC_FAIL(PSPH_ENOMEM, "oom (bucket occupancy)");
+  int rc = 0;
+  for (int bz = 0; bz < st->nbk[2] && !rc; bz++)
+    for (int by = 0; by < st->nbk[1] && !rc; by++)
+      for (int bx = 0; bx < st->nbk[0] && !rc; bx++) {
+        const size_t q = orc_bidx(st, bx, by, bz);
+        int bt[6], type;
+        orc_bucket_type(cfg, bx, by, bz, bt, &type);
+        if (!(type == ORC_MIXED && bt[4] == -1) || !(occ[q] & 1) || (occ[q] & 8) || st->berupt[q]) continue;
+        rc = orc_bucket_lattice(cfg, st, bx, by, bz, orc_lat_wghost_below, NULL);
+        if (!rc && bz > 0) rc = orc_bucket_lattice(cfg, st, bx, by, bz - 1, orc_lat_wghost, NULL);
+      }
+  free(occ);
+  return rc;
+}
+
+int orc_domain_adapt(const psph_config *cfg, orc_state *st) {      /* main.cc:256-304 */
+  if (!st->bval) ORC_FAIL(PSPH_EINVAL, "domain adaptation: call orc_domain_init first");
+  const size_t n0 = st->n;
+  uint8_t *occ = orc_bucket_occupancy(cfg, st);
+  if (!occ) ORC_FAIL(PSPH_ENOMEM, "oom (bucket occupancy)");
+  /* adapt_domain (adapt_domain.cc:41-194) */
+  const size_t nb = orc_nbuckets(st);
+  uint8_t *conv = calloc(nb ? nb : 1, 1);
+  if (!conv) { free(occ); ORC_FAIL(PSPH_ENOMEM, "oom"); }
+  for (int bz = 0; bz < st->nbk[2]; bz++)
+    for (int by = 0; by < st->nbk[1]; by++)
+      for (int bx = 0; bx < st->nbk[0]; bx++) {
+        const size_t q = orc_bidx(st, bx, by, bz);
+        int bt[6], type;
+        orc_bucket_type(cfg, bx, by, bz, bt, &type);
+        if (st->bval[q] || !(occ[q] & 1) || type == ORC_UNDERGROUND || type == ORC_PRESS_BC) continue;
+        if (!orc_any_neighbour(st, bx, by, bz, 2)) continue;
+        conv[q] = (uint8_t)(type == ORC_MIXED ? (bt[4] == -1 ? 2 : 3) : 1);
+      }
+  const double TINY = 1.0E-08;
+  for (size_t i = 0; i < st->n; i++) {
+    int b[3];
+    if (orc_bucket_of(cfg, st, i, b)) continue;
+    const size_t q = orc_bidx(st, b[0], b[1], b[2]);
+    if (!conv[q]) continue;
+    int turn = 1;
+    if (conv[q] == 2) turn = st->bc[i] == 1;              /* ground MIXED: pressure ghosts only */
+    else if (conv[q] == 3) {                               /* MIXED elsewhere: not outside dom   */
+      int bt[6], type;                                     /* (Bucket::determine_escape,        */
+      orc_bucket_type(cfg, b[0], b[1], b[2], bt, &type);   /*  bucket.cc:435-452)               */
+      for (int k = 0; k < 3 && turn; k++) {
+        if (bt[2 * k] == -1) { if ((st->x[k][i] - cfg->dom_lo[k]) < -TINY) turn = 0; }
+        else if (bt[2 * k + 1] == 1) { if ((st->x[k][i] - cfg->dom_hi[k]) > TINY) turn = 0; }
+      }
+    }
+    if (turn) { st->bc[i] = 100; st->inv[i] = 1; }
+  }
+  for (size_t q = 0; q < nb; q++) if (conv[q]) st->bval[q] = 1;
+  free(conv);
+  free(occ);
+  int rc = orc_add_pressure_ghost(cfg, st);
+  if (!rc) rc = orc_add_wall_ghost(cfg, st);
+  st->last_adapted = (int64_t)(st->n - n0);
+  if (!rc) orc_sort_canonical(cfg, st);
+  return rc;
+}
+
+int orc_domain_setup(const psph_config *cfg, const orc_meteo *met, uint32_t seed, double jitter,
+                     orc_state *st) {
+  int rc = orc_domain_init(cfg, met, seed, jitter, st);
+  if (rc) return rc;
+  /* add_air (add_air.cc:491-813): the potentially involved buckets, real  */
+  for (int bz = 0; bz < st->nbk[2] && !rc; bz++)
+    for (int by = 0; by < st->nbk[1] && !rc; by++)
+      for (int bx = 0; bx < st->nbk[0] && !rc; bx++) {
+        if (!st->bval[orc_bidx(st, bx, by, bz)]) continue;
+        int bt[6], type;
+        orc_bucket_type(cfg, bx, by, bz, bt, &type);
+        if (type == ORC_MIXED)
+          rc = orc_bucket_lattice(cfg, st, bx, by, bz, bt[4] == -1 ? orc_lat_real_above : orc_lat_real, NULL);
+        else if (type == ORC_OVERGROUND)
+          rc = orc_bucket_lattice(cfg, st, bx, by, bz, orc_lat_real, NULL);
+      }
+  if (!rc) rc = orc_add_pressure_ghost(cfg, st);          /* main.cc:128-132 */
+  if (!rc) rc = orc_add_wall_ghost(cfg, st);
+  if (!rc) rc = orc_setup_erupt(cfg, st);
+  if (!rc) orc_sort_canonical(cfg, st);
+  return rc;
+}
+
 int orc_step(const psph_config *cfg, orc_state *st) {
   int rc;
   double dt = orc_timestep(cfg, st);
   orc_incrtime(cfg, st, &dt);
   st->last_injected = st->last_removed = 0;
+  st->last_adapted = 0;
+  if (cfg->dim == 3 && cfg->adapt_domain && st->adapt_pending) {
+    st->adapt_pending = 0;
+    if ((rc = orc_domain_adapt(cfg, st))) return rc;
+    if (cfg->wall_bc && (rc = orc_apply_bcond(cfg, st))) return rc;
+  }
   if ((rc = orc_neighbors(cfg, st))) return rc;
   if (cfg->sml_update_int > 0 && st->step % cfg->sml_update_int == 0)
     if ((rc = orc_adaptive_sml(cfg, st))) return rc;
@@ -1268,6 +1626,7 @@ int orc_step(const psph_config *cfg, orc_state *st) {
   if (cfg->dim == 3) {
     if ((rc = orc_remove_escaped(cfg, st))) return rc;
     if (orc_iferupt(cfg, st) && (rc = orc_inject(cfg, st, dt))) return rc;
+    if (cfg->adapt_domain && (rc = orc_domain_scan(cfg, st))) return rc;
     if (cfg->wall_bc && (rc = orc_apply_bcond(cfg, st))) return rc;
     orc_sort_canonical(cfg, st);
   }

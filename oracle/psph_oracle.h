@@ -41,6 +41,16 @@ typedef struct orc_state {
   int ioutput;
   /* counters of the last step */
   int64_t last_injected, last_removed, last_pairs;
+  /* domain adaptation (ADJUST_DOMAIN), set up by orc_domain_init */
+  uint8_t *bval;            /* has_involved of each bucket: bit 0 potentially
+                               involved, bit 1 involved (bucket.h:400-414)  */
+  uint8_t *berupt;          /* erupt_flag of each bucket (setup_erupt.cc:221-253) */
+  int32_t nbk[3];           /* buckets per axis                              */
+  int32_t adapt_pending;    /* scan_outside_layer's count of the last step   */
+  uint32_t lat_seed;        /* jitter of the lattice the adaptation fills     */
+  double lat_jitter;
+  orc_meteo met;            /* atmosphere of the particles it adds           */
+  int64_t last_adapted;     /* particles the last adaptation added           */
 } orc_state;
 
 /* kernel (sph_lib.cc:207-253) */
@@ -92,6 +102,17 @@ int orc_remove_escaped(const psph_config *cfg, orc_state *st);      /* update_po
 int orc_inject(const psph_config *cfg, orc_state *st, double dt);   /* setup_erupt.cc:389-670 */
 int orc_apply_bcond(const psph_config *cfg, orc_state *st);         /* bcond.cc:24-249 */
 int orc_step(const psph_config *cfg, orc_state *st);                /* main.cc:217-486, 676-751 */
+/* domain adaptation: bucket flags of the state (dataread.cc:326-457,
+ * setup_erupt.cc:221-253); the initial state of an adaptive run (add_air.cc:
+ * 491-813, add_pressure_ghost.cc, add_wall_ghost.cc, setup_erupt.cc); one
+ * scan (scan_outside_layer.cc:44-121) and one adaptation (main.cc:256-304) */
+int orc_domain_init(const psph_config *cfg, const orc_meteo *met, uint32_t seed, double jitter,
+                    orc_state *st);
+int orc_domain_setup(const psph_config *cfg, const orc_meteo *met, uint32_t seed, double jitter,
+                     orc_state *st);
+int orc_domain_scan(const psph_config *cfg, orc_state *st);
+int orc_domain_adapt(const psph_config *cfg, orc_state *st);
+void orc_bucket_type(const psph_config *cfg, int bx, int by, int bz, int bt[6], int *type);
 int orc_iferupt(const psph_config *cfg, const orc_state *st);       /* properties.h:126-135 */
 void orc_conservation(const orc_state *st, double out[7]);
 double orc_diag_height(const orc_state *st);

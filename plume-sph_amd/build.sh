@@ -8,8 +8,10 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch=gfx950 -Xclang -target-feature -Xclan
        -ffp-contract=fast -Wall -Wno-unused-function -Wno-unused-variable -Wno-unused-result -Wno-unused-value)
 mkdir -p "$HERE/_build"
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/psph_host.cpp" -o "$HERE/_build/psph_host.o"
+"$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/psph_domain.cpp" -o "$HERE/_build/psph_domain.o"
 "$HIPCC" "${FLAGS[@]}" -c "$HERE/csrc/psph_device.hip" -o "$HERE/_build/psph_device.o"
-"$HIPCC" -shared --offload-arch=gfx950 "$HERE/_build/psph_host.o" "$HERE/_build/psph_device.o" -o "$OUT.tmp"
+"$HIPCC" -shared --offload-arch=gfx950 "$HERE/_build/psph_host.o" "$HERE/_build/psph_domain.o" \
+    "$HERE/_build/psph_device.o" -o "$OUT.tmp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"
 # Output library (H5Part / CSV): host code against the image's HDF5 and libpsph.so.

@@ -799,7 +799,7 @@ struct Adjoint {
   size_t lam_cap = 0, l_cap = 0;
   double *L[L_N] = {};        // by particle id
   double *l[L_N] = {};        // cell order of the step being reversed
-  double *aux[5] = {};        // per-particle seeds of one pass
+  double *aux[8] = {};        // per-particle seeds of one pass
   double *part = nullptr;     // reduction partials
   // chunks of adjoint rows
   uint32_t *flags = nullptr, *tpos = nullptr, *rows = nullptr, *cchunks = nullptr, *coff = nullptr;
@@ -839,10 +839,10 @@ static void snap_free(Snapshot &s) {
 
 static void rec_free(StepRec &r) {
   for (auto &p : r.d) { if (p) hipFree(p); p = nullptr; }
-  void *ptrs[] = {r.pid, r.refb, r.ckey, r.gflag, r.bc, r.phase, r.cs, r.toff, r.tbox};
+  void *ptrs[] = {r.pid, r.refb, r.ckey, r.gflag, r.bc, r.phase, r.bcn, r.cs, r.toff, r.tbox};
   for (auto p : ptrs) if (p) hipFree(p);
   r.pid = r.refb = r.ckey = r.gflag = nullptr;
-  r.bc = r.phase = nullptr;
+  r.bc = r.phase = r.bcn = nullptr;
   r.cs = r.toff = nullptr;
   r.tbox = nullptr;
   r.cap = r.cs_cap = r.tile_cap = r.n = 0;
@@ -930,11 +930,14 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
       for (auto pp : u32s) { if (*pp) hipFree(*pp); *pp = nullptr; }
       if (r.bc) hipFree(r.bc);
       if (r.phase) hipFree(r.phase);
-      r.bc = r.phase = nullptr;
-      for (auto &p : r.d) PSPH_TRY(dmalloc((void **)&p, cap * 8));
+      if (r.bcn) hipFree(r.bcn);
+      r.bc = r.phase = r.bcn = nullptr;
+      const bool wall = c->cfg.wall_bc && c->cfg.dim == 3;
+      for (int f = 0; f < (wall ? RF_N : RF_NBASE); f++) PSPH_TRY(dmalloc((void **)&r.d[f], cap * 8));
       for (auto pp : u32s) PSPH_TRY(dmalloc((void **)pp, cap * 4));
       PSPH_TRY(dmalloc((void **)&r.bc, cap * 4));
       PSPH_TRY(dmalloc((void **)&r.phase, cap * 4));
+      if (wall) PSPH_TRY(dmalloc((void **)&r.bcn, cap * 4));
       r.cap = cap;
     }
     const size_t ncs = (size_t)c->grid.ncells + 1;
@@ -977,9 +980,28 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
     PSPH_TRY(cp(RF_E1, c->f.e));
   } else if (stage == 2) {
     PSPH_TRY(cp(RF_RHO2, c->f.rho)); PSPH_TRY(cp(RF_E2, c->f.e)); PSPH_TRY(cp(RF_XI2, c->f.xi));
-  } else {
+  } else if (stage == 3) {
     PSPH_TRY(cp(RF_SV2X, c->f.svx)); PSPH_TRY(cp(RF_SV2Y, c->f.svy)); PSPH_TRY(cp(RF_SV2Z, c->f.svz));
     r.erupting = iferupt(c);
+  } else {
+    // 4: before the wall-ghost refresh (after update_pos and the inlet, whose
+    // particles come after the first r.n); 5: after it
+    if (!r.d[RF_BW] || !r.bcn) return fail(PSPH_EINVARIANT, "record: no wall-ghost fields");
+    const size_t m = r.n;
+    auto cpm = [&](int f, const double *src) -> int {
+      if (m) HIP_TRY(hipMemcpyAsync(r.d[f], src, m * 8, hipMemcpyDeviceToDevice, st));
+      return 0;
+    };
+    if (stage == 4) {
+      PSPH_TRY(cpm(RF_XN, c->f.x)); PSPH_TRY(cpm(RF_YN, c->f.y)); PSPH_TRY(cpm(RF_ZN, c->f.z));
+      if (m) {
+        HIP_TRY(hipMemcpyAsync(r.bcn, c->f.bc, m * 4, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_fill_f64, blocks(m), 256, 0, st, r.d[RF_BW], -1.0, m);
+      }
+    } else {
+      PSPH_TRY(cpm(RF_BRHO, c->f.rho)); PSPH_TRY(cpm(RF_BE, c->f.e)); PSPH_TRY(cpm(RF_BXI, c->f.xi));
+      PSPH_TRY(cpm(RF_BVX, c->f.vx)); PSPH_TRY(cpm(RF_BVY, c->f.vy)); PSPH_TRY(cpm(RF_BVZ, c->f.vz));
+    }
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -1109,6 +1131,167 @@ struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };
 
 // vent state derivatives (parameters.h:958-975): rho_v = p / (Rg T) / ng0,
 // e_v = Cvm(1) T, m_v = rho_v V / np, T = e_v / Cvm(1) = T_v, Cp = Cvm(1)
+// Wall-ghost refresh reverse (bcond.cc:24-249, as k_bcond computes it).  A
+// refreshed ghost i holds u_k = N_k / Wn: N_rho = sum w_j, N_v = sum w_j v_j /
+// rho_j, N_e = sum w_j e_j / rho_j, N_xi = the same sum of xi_j over the
+// neighbour buckets only, Wn = sum w_j / rho_j, with w_j = m_j W(r_i - x_j,
+// h_j) at its image r_i; then rho = u_rho, v = R u_v (R the mirror), e = u_e,
+// xi = u_xi and the EOS.  Per ghost, Lambda_k (the adjoint of u_k, the EOS
+// folded in) gives a_k = Lambda_k / Wn and b = -sum_k Lambda_k u_k / Wn (the
+// adjoint of Wn); per contributor j, lambda_w = a_rho + (a_v.v_j + a_e e_j +
+// [nbr] a_xi xi_j + b) / rho_j, and v_j, e_j, xi_j, rho_j, m_j and x_j
+// (through W) take their shares.  The refreshed fields are overwritten, so
+// the adjoints of the ghost's values before the refresh restart at zero.
+// Contributors gather their ghosts (no atomics): a contributor's candidates
+// are the wall ghosts around its mirror point, found in the step's cell grid
+// (wall ghosts do not move).  Pressure ghosts contribute to the refresh but
+// hold constant states, so their adjoints are not formed.
+enum { BA_ARHO, BA_AVX, BA_AVY, BA_AVZ, BA_AE, BA_AXI, BA_B, BA_GNG0, BA_N };
+struct BcondAdj { double *f[BA_N]; };
+
+__global__ void k_adj_bcond_ghost(uint32_t n, BcondArgs g, const double *gx, const double *gy, const double *gz,
+                                  const double *wrec, const double *brho, const double *bvx, const double *bvy,
+                                  const double *bvz, const double *be, const double *bxi, LamPtrs l, BcondAdj o) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  o.f[BA_GNG0][i] = 0;
+  const double wn = wrec[i];
+  if (!(wn >= 0)) return;                   // not refreshed: the adjoints pass through
+  const double lr = l.f[L_RHO][i], le = l.f[L_E][i], lxi = l.f[L_XI][i];
+  const double lv[3] = {l.f[L_VX][i], l.f[L_VY][i], l.f[L_VZ][i]};
+  const double lp = l.f[L_P][i], lc = l.f[L_C][i], lT = l.f[L_T][i], lcp = l.f[L_CP][i];
+  l.f[L_RHO][i] = l.f[L_E][i] = l.f[L_XI][i] = 0;
+  l.f[L_VX][i] = l.f[L_VY][i] = l.f[L_VZ][i] = 0;
+  l.f[L_P][i] = l.f[L_C][i] = l.f[L_T][i] = 0;
+  if (g.eos.lans) l.f[L_CP][i] = 0;          // Cp is an EOS output under LANS only
+  if (!(wn > 0)) return;                    // no contributor: a constant state
+  const double coord[3] = {gx[i], gy[i], gz[i]};
+  const double refc[3] = {coord[0], coord[1], 2 * g.zg - coord[2]};
+  double nrm[3];
+  mirror_normal(coord, refc, nrm);
+  const double v[3] = {bvx[i], bvy[i], bvz[i]};
+  const double vn = v[0] * nrm[0] + v[1] * nrm[1] + v[2] * nrm[2];
+  const double ln = lv[0] * nrm[0] + lv[1] * nrm[1] + lv[2] * nrm[2];
+  double u[3], lu[3];
+  for (int k = 0; k < 3; k++) { u[k] = v[k] - 2 * vn * nrm[k]; lu[k] = lv[k] - 2 * ln * nrm[k]; }
+  const double rho = brho[i], e = be[i], xi = bxi[i];
+  double grho, ge, gxi, gn;
+  eos_adj(g.eos, rho, e, xi, lp, lc, lT, lcp, grho, ge, gxi, gn);
+  const double Lr = lr + grho, Le = le + ge, Lxi = lxi + gxi;
+  const double iw = 1.0 / wn;
+  o.f[BA_ARHO][i] = Lr * iw;
+  o.f[BA_AVX][i] = lu[0] * iw; o.f[BA_AVY][i] = lu[1] * iw; o.f[BA_AVZ][i] = lu[2] * iw;
+  o.f[BA_AE][i] = Le * iw;
+  o.f[BA_AXI][i] = Lxi * iw;
+  o.f[BA_B][i] = -(Lr * rho + lu[0] * u[0] + lu[1] * u[1] + lu[2] * u[2] + Le * e + Lxi * xi) * iw;
+  o.f[BA_GNG0][i] = gn;
+}
+
+struct BcondAdjIn {
+  const double *xn, *yn, *zn;       // positions at the refresh (after update_pos)
+  const int32_t *bcn, *bc0;         // bc at the refresh and at the step's search
+  const double *gx, *gy, *gz;       // positions at the search (wall ghosts do not move)
+  const double *h, *m, *rho, *vx, *vy, *vz, *e, *xi;   // the values the refresh read
+  const double *wrec;
+  const uint32_t *cs;
+  GridDesc grid;
+  double h_new;                      // h of an erupt ghost that turned real in update_pos
+  double rfac;                       // per-axis reach over h (both support tests, with margin)
+};
+
+__global__ void k_adj_bcond_contr(uint32_t n, BcondArgs g, BcondAdjIn in, BcondAdj cf, LamPtrs l) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = in.bcn[j];
+  if (b != 100 && b != 99) return;
+  const double pj[3] = {in.xn[j], in.yn[j], in.zn[j]};
+  int bj[3];
+  if (!bucket_of(g, pj, bj)) return;
+  // images lie in the ground bucket layer or the one above it
+  if (bj[2] > g.bzg + 2) return;
+  const double hj = in.bc0[j] == 0 ? in.h_new : in.h[j];
+  const double mj[3] = {pj[0], pj[1], 2 * g.zg - pj[2]};
+  const double reach = in.rfac * hj;
+  const int nn[3] = {in.grid.nx, in.grid.ny, in.grid.nz};
+  int c0[3], c1[3];
+  for (int k = 0; k < 3; k++) {
+    if (in.grid.inv_e > 0) {
+      c0[k] = cell_of(mj[k] - reach, in.grid.lo[k], in.grid.inv_e, nn[k]);
+      c1[k] = cell_of(mj[k] + reach, in.grid.lo[k], in.grid.inv_e, nn[k]);
+    } else {
+      c0[k] = c1[k] = 0;
+    }
+  }
+  const double ir = 1.0 / in.rho[j], mj_ = in.m[j];
+  const double vj[3] = {in.vx[j], in.vy[j], in.vz[j]}, ej = in.e[j], xij = in.xi[j];
+  const double ih = 1.0 / hj, nk = kInvSqrtPi * ih, nk3 = nk * nk * nk;
+  double lvx = 0, lvy = 0, lvz = 0, le = 0, lxi = 0, lrho = 0, lm = 0, lx[3] = {0, 0, 0};
+  for (int cz = c0[2]; cz <= c1[2]; cz++)
+    for (int cy = c0[1]; cy <= c1[1]; cy++)
+      for (int cx = c0[0]; cx <= c1[0]; cx++) {
+        const uint32_t cell = ((uint32_t)cz * in.grid.ny + cy) * in.grid.nx + cx;
+        for (uint32_t q = in.cs[cell], qe = in.cs[cell + 1]; q < qe; q++) {
+          if (in.bc0[q] != 2 || !(in.wrec[q] > 0)) continue;
+          const double refc[3] = {in.gx[q], in.gy[q], 2 * g.zg - in.gz[q]};
+          int bi[3];
+          if (!bucket_of(g, refc, bi)) continue;
+          if (abs(bj[0] - bi[0]) > 1 || abs(bj[1] - bi[1]) > 1 || abs(bj[2] - bi[2]) > 1) continue;
+          const double amax = fmax(fabs(refc[0] - pj[0]), fmax(fabs(refc[1] - pj[1]), fabs(refc[2] - pj[2])));
+          if (!(amax <= g.cutoff2 * hj) || g.cut.outside(amax, hj)) continue;
+          const double dx[3] = {refc[0] - pj[0], refc[1] - pj[1], refc[2] - pj[2]};
+          double expt = 0;
+          for (int k = 0; k < 3; k++) { const double sk = dx[k] * ih; expt += sk * sk; }
+          const double w = nk3 * exp(-expt);
+          const double mw = mj_ * w, mwr = mw * ir;
+          const bool nbr = bj[0] != bi[0] || bj[1] != bi[1] || bj[2] != bi[2];
+          const double av[3] = {cf.f[BA_AVX][q], cf.f[BA_AVY][q], cf.f[BA_AVZ][q]};
+          const double ae = cf.f[BA_AE][q], axi = nbr ? cf.f[BA_AXI][q] : 0.0;
+          const double s = av[0] * vj[0] + av[1] * vj[1] + av[2] * vj[2] + ae * ej + axi * xij + cf.f[BA_B][q];
+          const double lw = cf.f[BA_ARHO][q] + s * ir;
+          lvx += av[0] * mwr; lvy += av[1] * mwr; lvz += av[2] * mwr;
+          le += ae * mwr;
+          lxi += axi * mwr;
+          lrho -= s * mwr * ir;
+          lm += lw * w;
+          const double gw = lw * mw * 2 * ih * ih;          // dW/dx_j = W 2 (r_i - x_j) / h^2
+          for (int k = 0; k < 3; k++) lx[k] += gw * dx[k];
+        }
+      }
+  l.f[L_VX][j] += lvx; l.f[L_VY][j] += lvy; l.f[L_VZ][j] += lvz;
+  l.f[L_E][j] += le; l.f[L_XI][j] += lxi; l.f[L_RHO][j] += lrho; l.f[L_M][j] += lm;
+  l.f[L_X][j] += lx[0]; l.f[L_Y][j] += lx[1]; l.f[L_Z][j] += lx[2];
+}
+
+static int adj_bcond(psph_ctx *c, Adjoint &A, const StepRec &R, ThetaGrad &th) {
+  const psph_config &k = c->cfg;
+  hipStream_t st = c->stream;
+  const uint32_t n = (uint32_t)R.n;
+  const BcondArgs g = bcond_geom(k);
+  BcondAdj cf;
+  for (int q = 0; q < BA_N; q++) cf.f[q] = A.aux[q];
+  const LamPtrs l = lam_ptrs(A.l);
+  hipLaunchKernelGGL(k_adj_bcond_ghost, blocks(n), 256, 0, st, n, g, R.d[RF_X], R.d[RF_Y], R.d[RF_Z], R.d[RF_BW],
+                     R.d[RF_BRHO], R.d[RF_BVX], R.d[RF_BVY], R.d[RF_BVZ], R.d[RF_BE], R.d[RF_BXI], l, cf);
+  double gn = 0;
+  PSPH_TRY(dev_sum(c, A, A.aux[BA_GNG0], n, &gn));
+  th.ng0 += gn;
+  BcondAdjIn in;
+  in.xn = R.d[RF_XN]; in.yn = R.d[RF_YN]; in.zn = R.d[RF_ZN];
+  in.bcn = R.bcn; in.bc0 = R.bc;
+  in.gx = R.d[RF_X]; in.gy = R.d[RF_Y]; in.gz = R.d[RF_Z];
+  in.h = R.d[RF_H]; in.m = R.d[RF_M]; in.rho = R.d[RF_RHO2];
+  in.vx = R.d[RF_V1X]; in.vy = R.d[RF_V1Y]; in.vz = R.d[RF_V1Z];
+  in.e = R.d[RF_E2]; in.xi = R.d[RF_XI2];
+  in.wrec = R.d[RF_BW];
+  in.cs = R.cs;
+  in.grid = R.grid;
+  in.h_new = k.ratio_sml_dx * k.h_air;
+  in.rfac = std::min(k.cutoff, k.cutoff2) * (1 + 1e-9) + 1e-12;
+  hipLaunchKernelGGL(k_adj_bcond_contr, blocks(n), 256, 0, st, n, g, in, cf, l);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
 struct VentDeriv { double drho_dT, drho_dn, de_dT, de_dn, dm_dT, dm_dn, dcp_dn; };
 static VentDeriv vent_deriv(const psph_config &k) {
   VentDeriv d;
@@ -1157,7 +1340,9 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
   if (!n) return 0;
   const LamPtrs Lid = lam_ptrs(A.L), l = lam_ptrs(A.l);
   hipLaunchKernelGGL(k_lam_gather, blocks(n), 256, 0, st, n, R.pid, Lid, l);
-  // 2. position update
+  // 2. wall-ghost refresh at the end of the step
+  if (k.wall_bc && k.dim == 3 && !bcond_trivial(k)) PSPH_TRY(adj_bcond(c, A, R, th));
+  // 3. position update
   hipLaunchKernelGGL(k_adj_pos, blocks(n), 256, 0, st, n, R.bc, R.erupting, k.lans, R.dt, l);
   PSPH_TRY(adj_rows(c, A, R));
   AdjCommon pc;
@@ -1171,7 +1356,7 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
   pc.cut = cut_test(k.cutoff);
   AdjVisits vl;
   PSPH_TRY(adj_visits(c, A, R, pc, vl));
-  // 3. velocity filter
+  // 4. velocity filter
   if (k.lans) {
     hipLaunchKernelGGL(k_adj_vsm_pre, blocks(n), 256, 0, st, n, R.bc, R.d[RF_WNV], R.d[RF_SV2X], R.d[RF_SV2Y],
                        R.d[RF_SV2Z], l, A.aux[0], A.aux[1], A.aux[2], A.aux[3]);
@@ -1179,7 +1364,7 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
                    A.aux[2], A.aux[3], R.bc, l};
     launch_adj<VsmAdj>(c, pc, vl, R, a);
   }
-  // 4. density + EOS
+  // 5. density + EOS
   {
     hipLaunchKernelGGL(k_adj_dens_pre, blocks(n), 256, 0, st, n, R.bc, R.d[RF_RHO2], R.d[RF_E2], R.d[RF_XI2],
                        R.d[RF_WNE], eos_const(k), k.energy_smooth, l, A.aux[0], A.aux[1], A.aux[2], A.aux[3], A.aux[4]);
@@ -1191,7 +1376,7 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
                     k.e_smooth_ratio * k.e_smooth_ratio == 16.0 ? 1 : 0, cut_test(k.cutoff), l};
     launch_adj<DensAdj>(c, pc, vl, R, a);
   }
-  // 5. momentum / energy
+  // 6. momentum / energy
   {
     hipLaunchKernelGGL(k_adj_rhs_pre, blocks(n), 256, 0, st, n, R.bc, R.dt, -k.g, k.lans, l,
                        A.aux[0], A.aux[1], A.aux[2], A.aux[3]);
@@ -1218,7 +1403,7 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
                                 double *grad_v, double *grad_sv, size_t n_cap) {
   if (!c || !out) return fail(PSPH_EINVAL, "null argument");
   const psph_config &k = c->cfg;
-  if (k.wall_bc) return fail(PSPH_ENOTSUP, "adjoint: the wall-ghost refresh (cfg.wall_bc) has no reverse; set it to 0");
+  if (k.adapt_domain) return fail(PSPH_ENOTSUP, "adjoint: domain adaptation (cfg.adapt_domain) has no reverse; set it to 0");
   if (k.dim != 3) return fail(PSPH_ENOTSUP, "adjoint: built for the 3D plume only");
   if (nsteps < 1 || ckpt_every < 1) return fail(PSPH_EINVAL, "adjoint: nsteps and ckpt_every must be >= 1");
   if (c->tape.mode) return fail(PSPH_EINVAL, "adjoint: a recorded run is already in progress");
@@ -1382,18 +1567,23 @@ extern "C" int psph_adjoint_run(psph_ctx *c, int64_t nsteps, int64_t ckpt_every,
 
 extern "C" int psph_replay_run(psph_ctx *c, int64_t nsteps) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
-  if (c->cfg.wall_bc) return fail(PSPH_ENOTSUP, "replay: the wall-ghost refresh (cfg.wall_bc) is not part of the recorded map");
+  if (c->cfg.adapt_domain) return fail(PSPH_ENOTSUP, "replay: domain adaptation is not part of the recorded map");
   Tape &tp = c->tape;
   if (tp.mode) return fail(PSPH_EINVAL, "replay: a recorded run is in progress");
   if (nsteps < 0 || nsteps > (int64_t)tp.dt.size()) return fail(PSPH_EINVAL, "replay: %lld steps, %zu recorded", (long long)nsteps, tp.dt.size());
   if (c->time != tp.t0 || c->step != tp.step0 || c->n != tp.n0)
     return fail(PSPH_EINVAL, "replay: load the state the recorded run started from (t %g, step %lld, %zu particles)",
                 tp.t0, (long long)tp.step0, tp.n0);
+  if ((int64_t)tp.xrec.size() < nsteps)
+    return fail(PSPH_EINVAL, "replay: positions of %zu steps recorded, %lld asked", tp.xrec.size(), (long long)nsteps);
   tp.mode = 2;
+  tp.frozen = true;
   tp.k = 0;
   int rc = 0;
   for (int64_t s = 0; s < nsteps && !rc; s++) rc = psph_step(c, nullptr);
+  if (!rc) rc = check_err(c, "replay");
   tp.mode = 0;
+  tp.frozen = false;
   return rc;
 }
 

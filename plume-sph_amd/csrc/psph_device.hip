@@ -32,11 +32,16 @@
 
 using namespace psph;
 
+// A failed HIP call also leaves the thread's last-error state set; it is
+// cleared here so that the error is reported once, by the return code, and not
+// again by the next unrelated check of another user of the runtime (torch).
 #define HIP_TRY(expr)                                                                        \
   do {                                                                                       \
     hipError_t e_ = (expr);                                                                  \
-    if (e_ != hipSuccess)                                                                    \
+    if (e_ != hipSuccess) {                                                                  \
+      (void)hipGetLastError();                                                               \
       return fail(PSPH_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    }                                                                                        \
   } while (0)
 
 #define PSPH_TRY(expr)            \
@@ -333,7 +338,9 @@ __device__ inline bool tile_meets(const ChunkBox &b, const float4 *tbox, uint32_
 }
 
 struct PairCommon {
-  const double *x, *y, *z;
+  const double *x, *y, *z;     // positions of the predicates (cells, boxes, tests)
+  const double *ax, *ay, *az;  // positions of the pair arithmetic (the same arrays
+                               // except in a frozen replay, Tape::xrec)
   const uint32_t *refb;
   const double *h_list;
   const int32_t *bc;
@@ -444,8 +451,12 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
                 const uint32_t q = sub + b * P::kSub;
                 double dx[3];
                 dx[0] = xi[0] - sx[q];
-                if (D == 3) { dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q]; }
-                P::pair(a, t, tile, q, base + q, i, dx);
+                double amax = fabs(dx[0]);
+                if (D == 3) {
+                  dx[1] = xi[1] - sy[q]; dx[2] = xi[2] - sz[q];
+                  amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
+                }
+                P::pair(a, t, tile, q, base + q, i, dx, amax);
                 if (pc.gflag && sgh[q]) pc.gflag[base + q] = 1u;
               }
             }
@@ -624,11 +635,17 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
 
 // A pair pass over the visit lists: stage each visited tile, test the pass's
 // exact kernel box and masks on the set bits, and run the pair arithmetic.
-template <int D, class P>
+struct NoArith {};
+struct ArithPos { double x[kWave], y[kWave], z[kWave]; };
+
+// FROZEN (a frozen replay): the predicates read pc.x (the recorded run's
+// positions) and the pair arithmetic pc.ax; otherwise both are pc.x.
+template <int D, class P, bool FROZEN = false>
 __global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, typename P::Args a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave];
   __shared__ uint8_t sgh[kWave];   // boundary ghost (bc 1, 2), one byte each
   __shared__ typename P::Tile tile;
+  __shared__ typename std::conditional<FROZEN, ArithPos, NoArith>::type sa;
   const uint32_t nch = *pc.nchunks;
   const uint32_t per = (nch + kXcds - 1) / kXcds;
   const uint32_t local = blockIdx.x / kXcds;
@@ -641,6 +658,8 @@ __global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, 
   const bool active = slot < cc;
   const uint32_t i = pc.targets[cf + (active ? slot : 0)];
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
+  double xa[3] = {xi[0], xi[1], xi[2]};
+  if constexpr (FROZEN) { xa[0] = pc.ax[i]; xa[1] = pc.ay[i]; xa[2] = pc.az[i]; }
   typename P::Tgt t;
   P::load_target(a, i, t);
   P::init(tile, lane);
@@ -653,6 +672,7 @@ __global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, 
     if ((uint32_t)lane < cnt) {
       const uint32_t j = base + lane;
       sx[lane] = pc.x[j]; sy[lane] = pc.y[j]; sz[lane] = pc.z[j];
+      if constexpr (FROZEN) { sa.x[lane] = pc.ax[j]; sa.y[lane] = pc.ay[j]; sa.z[lane] = pc.az[j]; }
       if (pc.gflag) { const int b = pc.bc[j]; sgh[lane] = b == 1 || b == 2; }
       P::stage(a, j, tile, lane);
     }
@@ -672,7 +692,11 @@ __global__ __launch_bounds__(kWave) void k_pairs_v(PairCommon pc, VisitList vl, 
           amax = fmax(amax, fmax(fabs(dx[1]), fabs(dx[2])));
         }
         if (pc.cut.outside(amax, P::hker(a, t, tile, q)) || !P::extra(a, t, tile, q, base + q, i)) continue;
-        P::pair(a, t, tile, q, base + q, i, dx);
+        if constexpr (FROZEN) {
+          dx[0] = xa[0] - sa.x[q];
+          if (D == 3) { dx[1] = xa[1] - sa.y[q]; dx[2] = xa[2] - sa.z[q]; }
+        }
+        P::pair(a, t, tile, q, base + q, i, dx, amax);
         if (pc.gflag && sgh[q]) pc.gflag[base + q] = 1u;
       }
     }
@@ -749,6 +773,29 @@ __device__ inline double exp_neg_tab(double e, const double *tab) {
   return tab[(int)k] * p;
 }
 
+// The exponential of the FP32-pair forms' FP64 weights.  FX false
+// (PSPH_FP32_PAIR): exp_neg_tab, FP64 throughout.  FX true (PSPH_FP32_EXP):
+// the FP32 hardware exponential 2^-tf of the argument t = e log2(e) formed in
+// FP64 and rounded once to tf, times 1 - (t - tf) ln 2, which takes the
+// rounding of t back out to second order: the relative error is the
+// instruction's own (about one FP32 ulp), independent from pair to pair,
+// while h, 1/h and the norm stay FP64.
+template <bool FX> constexpr int wtab_size() { return FX ? 1 : kETab; }
+template <bool FX> __device__ inline void wtab_fill(double *tab, int lane) {
+  if constexpr (!FX) etab_fill(tab, lane);
+}
+template <bool FX> __device__ inline double exp_neg_w(double e, const double *tab) {
+  if constexpr (!FX) {
+    return exp_neg_tab(e, tab);
+  } else {
+    constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
+    const double t = e * kLog2e;
+    const float tf = (float)t;
+    const double E = (double)__builtin_amdgcn_exp2f(-tf);
+    return fma(-E * kLn2, t - (double)tf, E);
+  }
+}
+
 template <class R> __device__ inline R pow16(R x) {
   x *= x; x *= x; x *= x;
   return x * x;
@@ -772,14 +819,14 @@ struct RhsArgs {
 // about 1 % of their size, and FP32 rounding there is what the velocities
 // feel.  The kernel value comes from the exp table (FP64 at FP32-like cost);
 // viscosity, heat conduction and the energy work terms stay FP32.
-struct RhsMixTile { double m64[kWave], pv64[kWave], etab[kETab]; };
+template <bool FX> struct RhsMixTile { double m64[kWave], pv64[kWave], etab[wtab_size<FX>()]; };
 struct NoTile {};
 
-template <class R, bool MIXED = std::is_same<R, float>::value>
+template <class R, bool MIXED = std::is_same<R, float>::value, bool FX = false>
 struct RhsPass {
   using Args = RhsArgs;
   static constexpr bool kMixed = MIXED;
-  struct Tile : std::conditional<kMixed, RhsMixTile, NoTile>::type {
+  struct Tile : std::conditional<kMixed, RhsMixTile<FX>, NoTile>::type {
     double hh[kWave];       // h / 2
     R m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
     R vx[kWave], vy[kWave], vz[kWave];
@@ -791,7 +838,7 @@ struct RhsPass {
     int bad;
   };
   __device__ static void init(Tile &s, int lane) {
-    if constexpr (kMixed) etab_fill(s.etab, lane);
+    if constexpr (kMixed) wtab_fill<FX>(s.etab, lane);
   }
   static constexpr int kSub = ::kSub;
   __device__ static void combine(Tgt &t) {
@@ -833,7 +880,7 @@ struct RhsPass {
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                const double *dxd) {
+                                const double *dxd, double) {
     t.npairs++;
     const R rhoj = s.rho[q];
     if (!(rhoj > R(0))) { t.bad = 1; return; }          // assert(uvecj[0] > 0)
@@ -914,7 +961,7 @@ struct RhsPass {
     for (int k = 0; k < D; k++) { r2 = fma(dxd[k], dxd[k], r2); dx[k] = (float)dxd[k]; }
     const double nk = kInvSqrtPi * ih64;
     const double norm = (D == 3) ? nk * nk * nk : nk;
-    const double G64 = -2.0 * ih2 * norm * exp_neg_tab(r2 * ih2, s.etab);
+    const double G64 = -2.0 * ih2 * norm * exp_neg_w<FX>(r2 * ih2, s.etab);
     const float G = (float)G64, dist_sq = (float)r2;
     const float Vj = s.V[q], rhoj = s.rho[q];
     float vij[3];
@@ -988,23 +1035,23 @@ struct DensArgs {
   int e_sq16;              // e_ratio^2 == 16: exp(-16 x) = exp(-x)^16 by four squarings
 };
 
-struct DensMixTile { double m64[kWave], irho64[kWave], e64[kWave], etab[kETab]; };
+template <bool FX> struct DensMixTile { double m64[kWave], irho64[kWave], e64[kWave], etab[wtab_size<FX>()]; };
 template <class R> struct DensTileR { R m[kWave], rho[kWave], e[kWave]; };
 
 // The FP32-pair form evaluates the density and energy-smoothing weights in
 // FP64 with the exp table: the pressure gradient of the next step amplifies
 // relative errors of rho and e by its cancellation.
-template <class R, bool MIXED = std::is_same<R, float>::value>
+template <class R, bool MIXED = std::is_same<R, float>::value, bool FX = false>
 struct DensPass {
   using Args = DensArgs;
   static constexpr bool kMixed = MIXED;
   // the mixed form stages only the FP64 copies it reads
-  struct Tile : std::conditional<kMixed, DensMixTile, DensTileR<R>>::type {
+  struct Tile : std::conditional<kMixed, DensMixTile<FX>, DensTileR<R>>::type {
     double hh[kWave];       // h / 2
     int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
   __device__ static void init(Tile &s, int lane) {
-    if constexpr (kMixed) etab_fill(s.etab, lane);
+    if constexpr (kMixed) wtab_fill<FX>(s.etab, lane);
   }
   struct Tgt { double hi, hh, supp, r1, r2, wn, en; };
   static constexpr int kSub = ::kSub;
@@ -1033,25 +1080,25 @@ struct DensPass {
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                const double *ds) {
+                                const double *ds, double amax) {
     const double hijd = t.hh + s.hh[q];
     if constexpr (kMixed) {
       const float ih32 = rrcp<float>((float)hijd);
       const double ih = (double)ih32 * fma(-hijd, (double)ih32, 2.0);
-      double r2 = 0, amax = 0;
+      double r2 = 0;
 #pragma unroll
-      for (int k = 0; k < D; k++) { r2 = fma(ds[k], ds[k], r2); amax = fmax(amax, fabs(ds[k])); }
+      for (int k = 0; k < D; k++) r2 = fma(ds[k], ds[k], r2);
       const double expt = r2 * (ih * ih);
       const double nk = kInvSqrtPi * ih;
       const double norm = (D == 3) ? nk * nk * nk : nk;
       const double mj = s.m64[q];
-      const double ex = exp_neg_tab(expt, s.etab);
+      const double ex = exp_neg_w<FX>(expt, s.etab);
       const double wm = norm * ex * mj;
       if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
       if (a.energy_smooth && !a.cut.outside(a.e_ratio * amax, hijd)) {
         const double nke = kInvSqrtPi * (a.e_ratio * ih);
         const double ne = (D == 3) ? nke * nke * nke : nke;
-        const double ee = a.e_sq16 ? pow16(ex) : exp_neg_tab(a.e_ratio * a.e_ratio * expt, s.etab);
+        const double ee = a.e_sq16 ? pow16(ex) : exp_neg_w<FX>(a.e_ratio * a.e_ratio * expt, s.etab);
         const double vr = ne * ee * mj * s.irho64[q];
         t.wn += vr;
         t.en += vr * s.e64[q];
@@ -1061,12 +1108,10 @@ struct DensPass {
     const R ih = rrcp<R>(hij);
     const R mj = s.m[q];
     R expt = 0;
-    double amax = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
       R sk = (R)ds[k] * ih;
       expt += sk * sk;
-      amax = fmax(amax, fabs(ds[k]));
     }
     const R nk = R(kInvSqrtPi) * ih;
     const R norm = (D == 3) ? nk * nk * nk : nk;
@@ -1125,18 +1170,18 @@ struct VsmArgs {
 // FP32 weights (relative error ~3e-7, dominated by the rounding of s^2 and of
 // h) made the velocity filter the largest source of FP32-vs-FP64 difference,
 // ~15x the RHS's FP32 terms.
-struct VsmMixTile { double mr64[kWave], v64[3][kWave], etab[kETab]; };
+template <bool FX> struct VsmMixTile { double mr64[kWave], v64[3][kWave], etab[wtab_size<FX>()]; };
 template <class R> struct VsmTileR { R mr[kWave], vx[kWave], vy[kWave], vz[kWave]; };
 
-template <class R>
+template <class R, bool FX = false>
 struct VsmPass {
   using Args = VsmArgs;
   static constexpr bool kMixed = std::is_same<R, float>::value;
-  struct Tile : std::conditional<kMixed, VsmMixTile, VsmTileR<R>>::type {
+  struct Tile : std::conditional<kMixed, VsmMixTile<FX>, VsmTileR<R>>::type {
     int ok[kWave];
   };
   __device__ static void init(Tile &s, int lane) {
-    if constexpr (kMixed) etab_fill(s.etab, lane);
+    if constexpr (kMixed) wtab_fill<FX>(s.etab, lane);
   }
   struct Tgt { double hi, supp, ih2, nrm, v64[3], dv[3], wn; R ih, norm, v[3]; };
   static constexpr int kSub = ::kSub;
@@ -1175,12 +1220,12 @@ struct VsmPass {
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                const double *ds) {
+                                const double *ds, double) {
     if constexpr (kMixed) {
       double r2 = 0;
 #pragma unroll
       for (int k = 0; k < D; k++) r2 = fma(ds[k], ds[k], r2);
-      const double w = t.nrm * exp_neg_tab(r2 * t.ih2, s.etab) * s.mr64[q];
+      const double w = t.nrm * exp_neg_w<FX>(r2 * t.ih2, s.etab) * s.mr64[q];
 #pragma unroll
       for (int k = 0; k < D; k++) t.dv[k] = fma(w, s.v64[k][q] - t.v64[k], t.dv[k]);
       t.wn += w;
@@ -1257,7 +1302,7 @@ struct AdaptPass {
   __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
-                                const double *dx) {
+                                const double *dx, double) {
     R expt = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -1312,7 +1357,7 @@ struct ListPass {
   __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t, uint32_t) { return true; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &, uint32_t, uint32_t j, uint32_t,
-                                const double *) {
+                                const double *, double) {
     if (a.mode) a.out[t.o + t.n] = j;
     t.n++;
   }
@@ -1324,9 +1369,11 @@ struct ListPass {
 // Binds the dimension into the pass's pair routine for the skeleton.
 template <int D, class P>
 struct Bound : P {
+  // dx: the pair's position difference; amax: max_k |dx_k| of the positions
+  // the predicates read (the same dx except in a frozen replay)
   __device__ static void pair(const typename P::Args &a, typename P::Tgt &t, const typename P::Tile &s,
-                             uint32_t q, uint32_t j, uint32_t i, const double *dx) {
-    P::template pair_d<D>(a, t, s, q, j, i, dx);
+                             uint32_t q, uint32_t j, uint32_t i, const double *dx, double amax) {
+    P::template pair_d<D>(a, t, s, q, j, i, dx, amax);
   }
 };
 
@@ -1357,7 +1404,19 @@ struct BcondArgs {
   int nb[3], bzg;
   CutTest cut;
   EosConst eos;
+  double *wrec;              // recording for the adjoint: Shepard normaliser of each refreshed ghost
+  uint32_t wrec_n;
+  const double *px, *py, *pz;   // positions the support tests read (x, y, z except in a frozen replay)
 };
+
+// unit normal of the mirror plane, from a ghost towards its image
+__device__ inline void mirror_normal(const double *coord, const double *refc, double *nrm) {
+  double d = 0;
+  for (int k = 0; k < 3; k++) { nrm[k] = refc[k] - coord[k]; d += nrm[k] * nrm[k]; }
+  if (!(d > 0)) { nrm[0] = nrm[1] = 0; nrm[2] = 1; d = 1; }
+  const double sd = sqrt(d);
+  for (int k = 0; k < 3; k++) nrm[k] /= sd;
+}
 
 __device__ inline bool bucket_of(const BcondArgs &a, const double *p, int *b) {
   bool ok = true;
@@ -1377,6 +1436,7 @@ __device__ inline bool bucket_of(const BcondArgs &a, const double *p, int *b) {
 __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], smw[kWave], sir[kWave];
   __shared__ double svx[kWave], svy[kWave], svz[kWave], se[kWave], sxi[kWave];
+  __shared__ double sqx[kWave], sqy[kWave], sqz[kWave];
   __shared__ int sbk[3][kWave];
   const int lane = threadIdx.x;
   const uint32_t i = blockIdx.x * kWave + lane;
@@ -1443,11 +1503,12 @@ __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
       bool ok = false;
       if (j < hi) {
         const int b = a.bc[j];
-        const double pj[3] = {a.x[j], a.y[j], a.z[j]};
+        const double qj[3] = {a.px[j], a.py[j], a.pz[j]};
         int bj[3];
-        ok = (b == 100 || b == 99 || b == 1) && bucket_of(a, pj, bj);   // contr_image (particle.h:562-565)
+        ok = (b == 100 || b == 99 || b == 1) && bucket_of(a, qj, bj);   // contr_image (particle.h:562-565)
         if (ok) {
-          sx[lane] = pj[0]; sy[lane] = pj[1]; sz[lane] = pj[2];
+          sx[lane] = a.x[j]; sy[lane] = a.y[j]; sz[lane] = a.z[j];
+          sqx[lane] = qj[0]; sqy[lane] = qj[1]; sqz[lane] = qj[2];
           sh[lane] = a.h[j]; smw[lane] = a.m[j]; sir[lane] = 1.0 / a.rho[j];
           svx[lane] = a.vx[j]; svy[lane] = a.vy[j]; svz[lane] = a.vz[j];
           se[lane] = a.e[j]; sxi[lane] = a.xi[j];
@@ -1465,7 +1526,7 @@ __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
           const bool near = abs(sbk[0][q] - bi[0]) <= 1 && abs(sbk[1][q] - bi[1]) <= 1 &&
                             abs(sbk[2][q] - bi[2]) <= 1;
           const double hj = sh[q];
-          const double amax = fmax(fabs(refc[0] - sx[q]), fmax(fabs(refc[1] - sy[q]), fabs(refc[2] - sz[q])));
+          const double amax = fmax(fabs(refc[0] - sqx[q]), fmax(fabs(refc[1] - sqy[q]), fabs(refc[2] - sqz[q])));
           // in_support; W = 0 outside the kernel box
           const bool in = near & (amax <= a.cutoff2 * hj) & !a.cut.outside(amax, hj);
           mask |= (uint64_t)in << q;
@@ -1496,11 +1557,9 @@ __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
     }
   }
   if (!act) return;
-  double nrm[3], d = 0;
-  for (int k = 0; k < 3; k++) { nrm[k] = refc[k] - coord[k]; d += nrm[k] * nrm[k]; }
-  if (!(d > 0)) { nrm[0] = nrm[1] = 0; nrm[2] = 1; d = 1; }
-  const double sd = sqrt(d);
-  for (int k = 0; k < 3; k++) nrm[k] /= sd;
+  if (a.wrec && i < a.wrec_n) a.wrec[i] = wnorm;
+  double nrm[3];
+  mirror_normal(coord, refc, nrm);
   if (wnorm > 0) {
     u0 /= wnorm; u1 /= wnorm; u2 /= wnorm; u3 /= wnorm; u4 /= wnorm;
     ks = ks / wnorm;
@@ -1621,6 +1680,31 @@ __global__ void k_h_by_id(uint32_t n, const uint32_t *pid, double *h, double *ta
   else tab[pid[i]] = h[i];
 }
 
+// positions of a step by particle id (rec: x block, y block, z block of nrec)
+__global__ void k_pos_to_rec(uint32_t n, const uint32_t *pid, const double *x, const double *y, const double *z,
+                             double *rec, uint32_t nrec) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t id = pid[i];
+  if (id >= nrec) return;
+  rec[id] = x[i]; rec[(size_t)nrec + id] = y[i]; rec[2 * (size_t)nrec + id] = z[i];
+}
+// and back, into the current order; a live particle without a record is an
+// error (the replay no longer follows the recorded run)
+__global__ void k_pos_from_rec(uint32_t n, const uint32_t *pid, const int32_t *bc, const double *rec,
+                               uint32_t nrec, const double *x, const double *y, const double *z, double *px,
+                               double *py, double *pz, int *err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t id = pid[i];
+  if (bc[i] < 0 || id >= nrec) {
+    if (bc[i] >= 0) raise_err(err, PSPH_EINVARIANT, i);
+    px[i] = x[i]; py[i] = y[i]; pz[i] = z[i];
+    return;
+  }
+  px[i] = rec[id]; py[i] = rec[(size_t)nrec + id]; pz[i] = rec[2 * (size_t)nrec + id];
+}
+
 // fixed-order block partial sums (deterministic)
 constexpr int kRedBlocks = 512;
 __global__ void k_conservation(uint32_t n, const int32_t *bc, const double *m, const double *e,
@@ -1688,6 +1772,20 @@ struct Tape {
   std::vector<std::vector<int32_t>> inlet;  // per step: (q, n) per vent column
   int32_t *tr_step = nullptr, *esc_step = nullptr;   // by id, -1 = never
   size_t id_cap = 0;
+  // Positions at each step's neighbour search, by particle id (x block, y
+  // block, z block of xrec_n[k] entries).  A replay with `frozen` set builds
+  // its cell order, tiles, visit lists and every pair predicate (kernel
+  // boxes, list membership, support) from these, and only the pair
+  // arithmetic from its own positions: the pair set and the summation order
+  // are the recorded run's, so a finite difference through the replay sees
+  // no kernel-box face crossings, as the adjoint does not.
+  std::vector<double *> xrec;
+  std::vector<uint32_t> xrec_n;
+  // with cfg.wall_bc, the same at each step's wall-ghost refresh (after
+  // update_pos): a frozen replay's refresh takes its contributor sets from them
+  std::vector<double *> xrecb;
+  std::vector<uint32_t> xrecb_n;
+  bool frozen = false;
 };
 
 // What the reverse sweep needs of one step, in that step's cell order.
@@ -1697,14 +1795,19 @@ enum RecField {
   RF_V1X, RF_V1Y, RF_V1Z, RF_E1,                         // after the RHS update
   RF_RHO2, RF_E2, RF_XI2, RF_WNE,                        // after the density pass
   RF_SV2X, RF_SV2Y, RF_SV2Z, RF_WNV,                     // after the velocity filter
+  // wall-ghost refresh (only with cfg.wall_bc): positions after update_pos,
+  // the refreshed ghost fields and each ghost's Shepard normaliser
+  RF_XN, RF_YN, RF_ZN, RF_BRHO, RF_BVX, RF_BVY, RF_BVZ, RF_BE, RF_BXI, RF_BW,
   RF_N
 };
+constexpr int RF_NBASE = RF_XN;                          // fields recorded in every step
 struct StepRec {
   size_t n = 0, cap = 0;
   double *d[RF_N] = {};
   uint32_t *pid = nullptr, *refb = nullptr, *ckey = nullptr;
   uint32_t *gflag = nullptr;                  // boundary ghosts met as neighbours in this step
   int32_t *bc = nullptr, *phase = nullptr;
+  int32_t *bcn = nullptr;                     // bc after update_pos (wall-ghost refresh only)
   uint32_t *cs = nullptr, *toff = nullptr;   // cell starts and first tile of each cell
   float4 *tbox = nullptr;                     // tile boxes
   size_t cs_cap = 0, tile_cap = 0;
@@ -1757,6 +1860,19 @@ struct psph_ctx {
   // persistent particle ids: load order, then injection order
   uint32_t *pid = nullptr, *pid_sp = nullptr;
   uint32_t next_pid = 0;
+  // positions the pair predicates read in a frozen replay (Tape::xrec)
+  double *px = nullptr, *py = nullptr, *pz = nullptr;
+  // domain adaptation (psph_domain_init, psph_domain.inl)
+  DomainBuckets dom;
+  bool dom_ready = false;
+  int dom_pending = 0;                 // scan_outside_layer's count of the last step
+  std::vector<double> met_z, met_data;
+  psph_meteo met{};
+  uint32_t dom_seed = 0;
+  double dom_jitter = 0;
+  uint32_t *d_occ = nullptr;
+  uint8_t *d_conv = nullptr, *d_sides = nullptr;
+  int64_t last_domain = 0;
   Tape tape;
   StepRec *rec = nullptr;      // set: psph_step captures the step for the reverse sweep
   Adjoint *adj = nullptr;
@@ -1790,7 +1906,11 @@ struct StageTimer {
 
 static int dmalloc(void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 8);
-  if (e != hipSuccess) return fail(PSPH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();             // reported here, not again by the next check
+    *p = nullptr;
+    return fail(PSPH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
   return 0;
 }
 
@@ -1819,9 +1939,12 @@ static int ensure_cub(psph_ctx *c, size_t bytes) {
 extern "C" int psph_create(const psph_config *cfg, size_t capacity, psph_ctx **out) {
   if (!cfg || !out) return fail(PSPH_EINVAL, "null argument");
   if (cfg->dim != 1 && cfg->dim != 3) return fail(PSPH_EINVAL, "dim must be 1 or 3");
-  if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_PAIR && cfg->precision != PSPH_FP32_PURE)
+  if (cfg->precision != PSPH_FP64 && cfg->precision != PSPH_FP32_PAIR && cfg->precision != PSPH_FP32_PURE &&
+      cfg->precision != PSPH_FP32_EXP)
     return fail(PSPH_EINVAL, "unknown precision form %d", cfg->precision);
-  if (cfg->kernel != PSPH_KERNEL_GAUSS_BOX) return fail(PSPH_ENOTSUP, "only the Gaussian box kernel is built");
+  if (cfg->kernel != PSPH_KERNEL_GAUSS_BOX)
+    return fail(PSPH_EINVAL, "kernel %d: the reference's one kernel, the Gaussian box (PSPH_KERNEL_GAUSS_BOX), "
+                             "is the only one", cfg->kernel);
   if (!(cfg->h_air > 0)) return fail(PSPH_EINVAL, "h_air must be > 0");
   {
     uint64_t bits;
@@ -1883,7 +2006,8 @@ extern "C" int psph_destroy(psph_ctx *c) {
   void *ptrs[] = {c->usp, c->isp, c->h_list, c->pid, c->pid_sp, c->refb, c->key_a, c->key_b, c->idx_a, c->idx_b,
                   c->flags, c->tpos, c->targets, c->cfirst, c->ccount, c->cs, c->tcnt, c->toff,
                   c->cchunks, c->coff, c->tbox, c->nchunks, c->cub, c->d_u64, c->d_err, c->d_part,
-                  c->vcount, c->voff, c->vnum, c->vmask, c->vent};
+                  c->vcount, c->voff, c->vnum, c->vmask, c->vent, c->px, c->py, c->pz,
+                  c->d_occ, c->d_conv, c->d_sides};
   for (auto p : ptrs) if (p) hipFree(p);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -1932,6 +2056,21 @@ extern "C" int psph_count(psph_ctx *c, size_t *n_total, size_t *n_real) {
   for (auto b : bc) { if (b >= 0) nt++; if (b == 100 || b == 99) nr++; }
   if (n_total) *n_total = nt;
   if (n_real) *n_real = nr;
+  return 0;
+}
+
+extern "C" int psph_device_view(psph_ctx *c, psph_soa *dev, size_t *n, void **stream) {
+  if (!c || !dev) return fail(PSPH_EINVAL, "null argument");
+  // finish pending work so that the caller may use the arrays on any stream
+  HIP_TRY(hipStreamSynchronize(c->stream));
+#define V_(a) dev->a = c->f.a;
+  PSPH_DFIELDS(V_)
+  V_(bc) V_(phase)
+#undef V_
+  dev->key0 = c->f.k0; dev->key1 = c->f.k1; dev->key2 = c->f.k2;
+  dev->involved = c->f.inv;
+  if (n) *n = c->n;
+  if (stream) *stream = (void *)c->stream;
   return 0;
 }
 
@@ -2091,8 +2230,21 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   const GridDesc &g = c->grid;
   uint32_t n = (uint32_t)c->n;
   hipStream_t s = c->stream;
+  Tape &tp = c->tape;
+  const bool frozen = tp.mode == 2 && tp.frozen;
+  if (frozen) {
+    if (tp.k >= (int64_t)tp.xrec.size()) return fail(PSPH_EINVAL, "replay: no recorded positions for step %lld", (long long)tp.k);
+    if (!c->px) {
+      PSPH_TRY(dmalloc((void **)&c->px, c->cap * 8));
+      PSPH_TRY(dmalloc((void **)&c->py, c->cap * 8));
+      PSPH_TRY(dmalloc((void **)&c->pz, c->cap * 8));
+    }
+    if (n) hipLaunchKernelGGL(k_pos_from_rec, blocks(n), 256, 0, s, n, c->pid, c->f.bc, tp.xrec[tp.k], tp.xrec_n[tp.k],
+                              c->f.x, c->f.y, c->f.z, c->px, c->py, c->pz, c->d_err);
+  }
   if (n) {
-    hipLaunchKernelGGL(k_cellkey, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.bc, g, c->key_a, c->idx_a);
+    hipLaunchKernelGGL(k_cellkey, blocks(n), 256, 0, s, n, frozen ? c->px : c->f.x, frozen ? c->py : c->f.y,
+                       frozen ? c->pz : c->f.z, c->f.bc, g, c->key_a, c->idx_a);
     int end_bit = 1;
     while (end_bit < 32 && ((uint64_t)1 << end_bit) <= ((uint64_t)g.ncells << (3 * g.fbits))) end_bit++;
     size_t bytes = 0;
@@ -2111,6 +2263,13 @@ extern "C" int psph_neighbors(psph_ctx *c) {
 #undef G_
     hipLaunchKernelGGL(k_gather<uint32_t>, blocks(n), 256, 0, s, n, c->idx_b, c->pid, c->pid_sp);
     std::swap(c->pid, c->pid_sp);
+    if (frozen) {
+      double **pp[3] = {&c->px, &c->py, &c->pz};
+      for (auto q : pp) {
+        hipLaunchKernelGGL(k_gather<double>, blocks(n), 256, 0, s, n, c->idx_b, *q, c->dsp[0]);
+        std::swap(*q, c->dsp[0]);
+      }
+    }
     std::swap(c->key_a, c->key_b);   // key_a: sorted keys
   }
   // drop removed particles (their key == ncells sorts them to the tail)
@@ -2127,14 +2286,24 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   // tiles of each cell and their boxes
   hipLaunchKernelGGL(k_tile_counts, blocks(g.ncells + 1), 256, 0, s, g.ncells, c->cs, c->tcnt);
   PSPH_TRY(excl_scan(c, c->tcnt, c->toff, (size_t)g.ncells + 1));
+  // the positions every predicate of this step reads
+  const double *qx = frozen ? c->px : c->f.x, *qy = frozen ? c->py : c->f.y, *qz = frozen ? c->pz : c->f.z;
+  if (tp.mode == 1 && (int64_t)tp.xrec.size() == tp.k) {
+    double *rec = nullptr;
+    const uint32_t nrec = std::max<uint32_t>(c->next_pid, 1);
+    PSPH_TRY(dmalloc((void **)&rec, (size_t)nrec * 3 * 8));
+    tp.xrec.push_back(rec);
+    tp.xrec_n.push_back(nrec);
+    if (n) hipLaunchKernelGGL(k_pos_to_rec, blocks(n), 256, 0, s, n, c->pid, c->f.x, c->f.y, c->f.z, rec, nrec);
+  }
   hipLaunchKernelGGL(k_tile_boxes, (unsigned)(n / kWave + g.ncells + 1), kWave, 0, s, g.ncells, c->cs, c->toff,
-                     c->f.x, c->f.y, c->f.z, c->tbox);
+                     qx, qy, qz, c->tbox);
   // reference buckets (list membership) and h at list time
   int nb[3] = {1, 1, 1};
   double del = c->cfg.bucket_edge > 0 ? c->cfg.bucket_edge : 1.0;
   if (c->cfg.dim == 3)
     for (int d = 0; d < 3; d++) nb[d] = (int)std::llround((c->cfg.ext_hi[d] - c->cfg.ext_lo[d]) / del);
-  if (n) hipLaunchKernelGGL(k_refb, blocks(n), 256, 0, s, n, c->f.x, c->f.y, c->f.z, c->f.h,
+  if (n) hipLaunchKernelGGL(k_refb, blocks(n), 256, 0, s, n, qx, qy, qz, c->f.h,
                             c->cfg.ext_lo[0], c->cfg.ext_lo[1], c->cfg.ext_lo[2], del, nb[0], nb[1], nb[2],
                             c->refb, c->h_list);
   // targets and their chunks (<= kChunk consecutive targets of one cell)
@@ -2166,7 +2335,9 @@ static Traversal traversal(psph_ctx *c, double reach) {
 
 static PairCommon make_common(psph_ctx *c, double reach) {
   PairCommon pc;
-  pc.x = c->f.x; pc.y = c->f.y; pc.z = c->f.z;
+  const bool frozen = c->tape.mode == 2 && c->tape.frozen;
+  pc.x = frozen ? c->px : c->f.x; pc.y = frozen ? c->py : c->f.y; pc.z = frozen ? c->pz : c->f.z;
+  pc.ax = c->f.x; pc.ay = c->f.y; pc.az = c->f.z;
   pc.refb = c->refb; pc.h_list = c->h_list;
   pc.bc = c->f.bc;
   pc.gflag = c->rec ? c->rec->gflag : nullptr;
@@ -2245,9 +2416,12 @@ static int ensure_visits(psph_ctx *c) {
 template <int D, class P>
 static void launch_visit_pass(psph_ctx *c, const PairCommon &pc, const VisitList &vl, const typename P::Args &a) {
   const unsigned nb = chunk_grid(c->n, c->grid.ncells, kChunk);
-  hipLaunchKernelGGL((k_pairs_v<D, Bound<D, P>>), nb, kWave, 0, c->stream, pc, vl, a);
+  if (c->tape.mode == 2 && c->tape.frozen)
+    hipLaunchKernelGGL((k_pairs_v<D, Bound<D, P>, true>), nb, kWave, 0, c->stream, pc, vl, a);
+  else
+    hipLaunchKernelGGL((k_pairs_v<D, Bound<D, P>>), nb, kWave, 0, c->stream, pc, vl, a);
 }
-template <class PM, class PP, class PD>
+template <class PM, class PX, class PP, class PD>
 static int launch_visits(psph_ctx *c, const typename PD::Args &a) {
   PSPH_TRY(ensure_visits(c));
   const PairCommon pc = make_common(c, c->reach_main);
@@ -2255,10 +2429,12 @@ static int launch_visits(psph_ctx *c, const typename PD::Args &a) {
   const int form = c->cfg.precision;
   if (c->cfg.dim == 3) {
     if (form == PSPH_FP32_PAIR) launch_visit_pass<3, PM>(c, pc, vl, a);
+    else if (form == PSPH_FP32_EXP) launch_visit_pass<3, PX>(c, pc, vl, a);
     else if (form == PSPH_FP32_PURE) launch_visit_pass<3, PP>(c, pc, vl, a);
     else launch_visit_pass<3, PD>(c, pc, vl, a);
   } else {
     if (form == PSPH_FP32_PAIR) launch_visit_pass<1, PM>(c, pc, vl, a);
+    else if (form == PSPH_FP32_EXP) launch_visit_pass<1, PX>(c, pc, vl, a);
     else if (form == PSPH_FP32_PURE) launch_visit_pass<1, PP>(c, pc, vl, a);
     else launch_visit_pass<1, PD>(c, pc, vl, a);
   }
@@ -2350,7 +2526,7 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 3);
-    if (n) PSPH_TRY((launch_visits<RhsPass<float>, RhsPass<float, false>, RhsPass<double>>(c, a)));
+    if (n) PSPH_TRY((launch_visits<RhsPass<float>, RhsPass<float, true, true>, RhsPass<float, false>, RhsPass<double>>(c, a)));
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.vx, c->dsp[1]); std::swap(c->f.vy, c->dsp[2]); std::swap(c->f.vz, c->dsp[3]);
@@ -2381,7 +2557,7 @@ extern "C" int psph_density(psph_ctx *c) {
   if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 4);
-    if (n) PSPH_TRY((launch_visits<DensPass<float>, DensPass<float, false>, DensPass<double>>(c, a)));
+    if (n) PSPH_TRY((launch_visits<DensPass<float>, DensPass<float, true, true>, DensPass<float, false>, DensPass<double>>(c, a)));
   }
   HIP_TRY(hipGetLastError());
   std::swap(c->f.rho, c->dsp[1]);
@@ -2402,7 +2578,7 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   if (c->n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 5);
-    if (c->n) PSPH_TRY((launch_visits<VsmPass<float>, VsmPass<float>, VsmPass<double>>(c, a)));
+    if (c->n) PSPH_TRY((launch_visits<VsmPass<float>, VsmPass<float, true>, VsmPass<float>, VsmPass<double>>(c, a)));
   }
   HIP_TRY(hipGetLastError());
   return check_err(c, "vsmooth");
@@ -2469,6 +2645,13 @@ static void tape_clear(Tape &tp) {
   tp.dt.clear();
   tp.hslot.clear();
   tp.inlet.clear();
+  for (double *p : tp.xrec) if (p) hipFree(p);
+  tp.xrec.clear();
+  tp.xrec_n.clear();
+  for (double *p : tp.xrecb) if (p) hipFree(p);
+  tp.xrecb.clear();
+  tp.xrecb_n.clear();
+  tp.frozen = false;
   tp.k = 0;
   tp.mode = 0;
 }
@@ -2514,26 +2697,43 @@ extern "C" int psph_inject(psph_ctx *c, double dt, int64_t *added) {
 
 // wall-ghost refresh over the cell grid of the last neighbour search; disp:
 // the largest per-axis displacement of any particle since that search
-static int apply_bcond_grid(psph_ctx *c, double disp) {
-  const psph_config &k = c->cfg;
-  if (k.dim != 3 || !c->n) return 0;
+// ground on a bucket face: no MIXED bucket, nothing to refresh
+static bool bcond_trivial(const psph_config &k) {
   const double fz = (k.dom_lo[2] - k.ext_lo[2]) / k.bucket_edge;
-  if (fz == std::floor(fz)) return 0;            // ground on a bucket face: no MIXED bucket
+  return fz == std::floor(fz);
+}
+
+// wrec (optional, wrec_n entries): receives each refreshed ghost's Shepard
+// normaliser, for the adjoint
+// the bucket geometry, ground, support tests and EOS of the refresh
+static BcondArgs bcond_geom(const psph_config &k) {
   BcondArgs a{};
-  a.x = c->f.x; a.y = c->f.y; a.z = c->f.z; a.h = c->f.h; a.m = c->f.m;
-  a.rho = c->f.rho; a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.e = c->f.e; a.xi = c->f.xi;
-  a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.gam = c->f.gam; a.cp = c->f.cp;
-  a.bc = c->f.bc; a.cs = c->cs; a.toff = c->toff; a.tbox = c->tbox; a.g = c->grid;
-  const double hm = std::max(c->h_max, k.ratio_sml_dx * k.h_air);
-  a.reach = k.cutoff * hm * (1 + 1e-9) + disp * (1 + 1e-9);
   a.zg = k.dom_lo[2]; a.del = k.bucket_edge; a.cutoff2 = k.cutoff2;
   for (int d = 0; d < 3; d++) {
     a.ext_lo[d] = k.ext_lo[d];
     a.nb[d] = (int)std::llround((k.ext_hi[d] - k.ext_lo[d]) / k.bucket_edge);
   }
-  a.bzg = (int)std::floor(fz);
+  a.bzg = (int)std::floor((k.dom_lo[2] - k.ext_lo[2]) / k.bucket_edge);
   a.cut = cut_test(k.cutoff);
   a.eos = eos_const(k);
+  return a;
+}
+
+// qx, qy, qz (optional): the positions the support tests read
+static int apply_bcond_grid(psph_ctx *c, double disp, double *wrec = nullptr, size_t wrec_n = 0,
+                            const double *qx = nullptr, const double *qy = nullptr, const double *qz = nullptr) {
+  const psph_config &k = c->cfg;
+  if (k.dim != 3 || !c->n || bcond_trivial(k)) return 0;
+  BcondArgs a = bcond_geom(k);
+  a.wrec = wrec;
+  a.wrec_n = (uint32_t)wrec_n;
+  a.x = c->f.x; a.y = c->f.y; a.z = c->f.z; a.h = c->f.h; a.m = c->f.m;
+  a.rho = c->f.rho; a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.e = c->f.e; a.xi = c->f.xi;
+  a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.gam = c->f.gam; a.cp = c->f.cp;
+  a.bc = c->f.bc; a.cs = c->cs; a.toff = c->toff; a.tbox = c->tbox; a.g = c->grid;
+  a.px = qx ? qx : c->f.x; a.py = qx ? qy : c->f.y; a.pz = qx ? qz : c->f.z;
+  const double hm = std::max(c->h_max, k.ratio_sml_dx * k.h_air);
+  a.reach = k.cutoff * hm * (1 + 1e-9) + disp * (1 + 1e-9);
   hipLaunchKernelGGL(k_bcond, blocks(c->n, kWave), kWave, 0, c->stream, (uint32_t)c->n, a);
   HIP_TRY(hipGetLastError());
   return 0;
@@ -2545,11 +2745,19 @@ extern "C" int psph_apply_bcond(psph_ctx *c) {
   return apply_bcond_grid(c, 0.0);
 }
 
+// domain adaptation (psph_domain.inl)
+namespace {
+int domain_adapt(psph_ctx *c);
+int domain_scan_step(psph_ctx *c);
+}  // namespace
+
 extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   const psph_config &k = c->cfg;
   Tape &tp = c->tape;
   double dt;
+  if (k.adapt_domain && k.dim == 3 && !c->dom_ready)
+    return fail(PSPH_EINVAL, "adapt_domain: call psph_domain_init after psph_load");
   if (tp.mode == 2) {
     if (tp.k >= (int64_t)tp.dt.size()) return fail(PSPH_EINVAL, "replay: past the end of the recorded run");
     dt = tp.dt[tp.k];
@@ -2559,6 +2767,17 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
     PSPH_TRY(psph_timestep(c, &dt));
     PSPH_TRY(psph_advance_time(c, &dt));
     if (tp.mode == 1) tp.dt.push_back(dt);
+  }
+  c->last_domain = 0;
+  if (k.adapt_domain && k.dim == 3 && c->dom_pending) {
+    // the region of real particles grows before this step's search
+    // (main.cc:256-304); the wall ghosts are refreshed after it
+    c->dom_pending = 0;
+    PSPH_TRY(domain_adapt(c));
+    if (k.wall_bc && c->n) {
+      PSPH_TRY(psph_neighbors(c));
+      PSPH_TRY(apply_bcond_grid(c, 0.0));
+    }
   }
   PSPH_TRY(psph_neighbors(c));
   if (tp.mode == 2) {
@@ -2592,6 +2811,7 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
     c->rec->inj_hi = c->next_pid;
     c->rec->inj_vel.assign(c->inj.vz.begin(), c->inj.vz.end());
   }
+  if (k.adapt_domain && k.dim == 3) PSPH_TRY(domain_scan_step(c));   // main.cc:433-436
   if (k.wall_bc && k.dim == 3 && c->n) {
     // apply_bcond at the end of the step (main.cc:468): the grid is the one of
     // this step's neighbour search; particles have moved by at most dt vmax
@@ -2603,7 +2823,33 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     double vmax;
     std::memcpy(&vmax, &bits, 8);
-    PSPH_TRY(apply_bcond_grid(c, std::fabs(dt) * vmax));
+    double disp = std::fabs(dt) * vmax;
+    const uint32_t nb = (uint32_t)c->n;
+    if (tp.mode == 1 && (int64_t)tp.xrecb.size() == tp.k) {
+      // positions at the refresh, by id, for a frozen replay
+      double *rec = nullptr;
+      const uint32_t nrec = std::max<uint32_t>(c->next_pid, 1);
+      PSPH_TRY(dmalloc((void **)&rec, (size_t)nrec * 3 * 8));
+      tp.xrecb.push_back(rec);
+      tp.xrecb_n.push_back(nrec);
+      hipLaunchKernelGGL(k_pos_to_rec, blocks(nb), 256, 0, c->stream, nb, c->pid, c->f.x, c->f.y, c->f.z, rec, nrec);
+    }
+    const bool frozen = tp.mode == 2 && tp.frozen;
+    if (frozen) {
+      // the contributor sets of the recorded run: support tests on its
+      // positions, whose displacement since the search differs from this
+      // run's by the perturbation (the reach takes 1% more)
+      if (tp.k >= (int64_t)tp.xrecb.size() || !c->px)
+        return fail(PSPH_EINVAL, "replay: no recorded refresh positions for step %lld", (long long)tp.k);
+      hipLaunchKernelGGL(k_pos_from_rec, blocks(nb), 256, 0, c->stream, nb, c->pid, c->f.bc, tp.xrecb[tp.k],
+                         tp.xrecb_n[tp.k], c->f.x, c->f.y, c->f.z, c->px, c->py, c->pz, c->d_err);
+      disp *= 1.01;
+    }
+    if (c->rec) PSPH_TRY(rec_capture(c, 4, dt));
+    PSPH_TRY(apply_bcond_grid(c, disp, c->rec ? c->rec->d[RF_BW] : nullptr, c->rec ? c->rec->n : 0,
+                              frozen ? c->px : nullptr, c->py, c->pz));
+    if (c->rec) PSPH_TRY(rec_capture(c, 5, dt));
+    if (frozen) PSPH_TRY(check_err(c, "replay refresh"));
   }
   if (tp.mode) tp.k++;
   // TimeProps::ifoutput (properties.h:104-120)
@@ -2620,6 +2866,7 @@ extern "C" int psph_step(psph_ctx *c, psph_step_out *out) {
     out->n_injected = added; out->n_removed = c->last_removed;
     out->n_pairs = (int64_t)c->last_pairs;
     out->output = output;
+    out->n_domain = c->last_domain;
   }
   return 0;
 }
@@ -2743,3 +2990,4 @@ extern "C" int psph_profile_read(psph_ctx *c, double ms[PSPH_NSTAGES], int64_t l
 }
 
 #include "psph_adjoint.inl"
+#include "psph_domain.inl"

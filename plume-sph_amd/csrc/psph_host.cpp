@@ -55,7 +55,7 @@ size_t HostParticles::push() {
 }
 void HostParticles::permute(const std::vector<uint32_t> &perm) {
   size_t n = perm.size();
-#define R(a) { auto t = a; for (size_t i = 0; i < n; i++) t[i] = a[perm[i]]; a.swap(t); }
+#define R(a) { decltype(a) t(n); for (size_t i = 0; i < n; i++) t[i] = a[perm[i]]; a.swap(t); }
   PSPH_FOR_D(R) PSPH_FOR_U(R) PSPH_FOR_I(R)
 #undef R
 }
@@ -294,9 +294,86 @@ int inject_host(const psph_config &cfg, double time, double dt, int64_t step, Ho
   return 0;
 }
 
+void lattice_point(const psph_config &cfg, uint32_t seed, double jitter, const int64_t l[3],
+                   const int64_t nl[3], double exact[3], double pos[3]) {
+  const double dx = cfg.h_air, dx2 = 0.5 * dx, del = cfg.bucket_edge;
+  const int per = (int)std::llround(del / dx);
+  for (int k = 0; k < 3; k++) {
+    double mincrd = cfg.ext_lo[k] + (double)(l[k] / per) * del;   // preprocess.cc:147-159
+    exact[k] = pos[k] = mincrd + dx2 + (double)(l[k] % per) * dx;   // add_air.cc:575-577
+  }
+  if (jitter > 0) {
+    uint64_t L = (uint64_t)((l[2] * nl[1] + l[1]) * nl[0] + l[0]);
+    uint32_t r[4] = {uint32_t(L), uint32_t(L >> 32), 0, 0};
+    philox(r, seed, 0x50535048u);
+    for (int k = 0; k < 3; k++) {
+      double u = ((double)r[k] + 0.5) * (1.0 / 4294967296.0);
+      pos[k] += (2.0 * u - 1.0) * jitter * dx;
+    }
+  }
+}
+
+size_t push_air(const psph_config &cfg, const psph_meteo *met, HostParticles &P, const double pc[3],
+                double h, int bc, int inv, uint32_t add_step) {
+  size_t i = P.push();
+  P.x[i] = pc[0]; P.y[i] = pc[1]; P.z[i] = pc[2];
+  double normc[3];
+  for (int k = 0; k < 3; k++) normc[k] = (pc[k] - cfg.ext_lo[k]) / (cfg.ext_hi[k] - cfg.ext_lo[k]);
+  uint32_t key[3];
+  thsfc3d(normc, add_step, key);
+  P.k0[i] = key[0]; P.k1[i] = key[1]; P.k2[i] = key[2];
+  P.h[i] = h;
+  P.bc[i] = bc; P.phase[i] = 1; P.inv[i] = inv; P.xi[i] = 0;
+  // initial_air + air_prop_meteo_based (sph_lib.cc:724-766, 831-872)
+  double sml2 = 0.5 * P.h[i];
+  double range[6] = {pc[0] - sml2, pc[0] + sml2, pc[1] - sml2, pc[1] + sml2, pc[2] - sml2, pc[2] + sml2};
+  double pressure = meteo_interp(met, pc[2] * 0.001, 1) * 100;
+  double density = meteo_interp(met, pc[2] * 0.001, 0);
+  double energy = pressure / (density * (cfg.gamma_p - 1));
+  double d1 = meteo_interp(met, range[4] * 0.001, 0);
+  double d2 = meteo_interp(met, range[5] * 0.001, 0);
+  double mass = (range[1] - range[0]) * (range[3] - range[2]) * (range[5] - range[4]) *
+                (0.1666667 * d1 + 0.6666666 * density + 0.1666667 * d2);
+  P.rho[i] = density; P.e[i] = energy; P.p[i] = pressure; P.m[i] = mass;
+  eos(cfg, density, energy, 0.0, P.p[i], P.c[i], P.T[i], P.gam[i], P.cp[i]);
+  return i;
+}
+
+void setup_vent(const psph_config &cfg, HostParticles &P) {
+  // every non-erupt particle inside the vent duct goes (setup_erupt.cc:302-320)
+  const double rvsq = cfg.vent_r * cfg.vent_r;
+  std::vector<uint32_t> keep;
+  keep.reserve(P.size());
+  for (size_t i = 0; i < P.size(); i++) {
+    double dist = 0;
+    dist += P.x[i] * P.x[i];
+    dist += P.y[i] * P.y[i];
+    if (P.bc[i] != PSPH_BC_ERUPT_GHOST && dist <= rvsq && P.z[i] <= cfg.vent_z && P.z[i] >= cfg.vent_bot) continue;
+    keep.push_back((uint32_t)i);
+  }
+  if (keep.size() != P.size()) P.permute(keep);
+  // the duct's erupt ghosts (setup_erupt.cc:105-205)
+  double z = cfg.vent_z - 0.5 * cfg.vent_sml;
+  while (z >= cfg.vent_bot) {
+    vent_columns(cfg, [&](double x, double y) {
+      double dist = 0;
+      dist += x * x;
+      dist += y * y;
+      if (dist <= rvsq) new_erupt(cfg, P, x, y, z, 0, vel_prof(cfg, cfg.vent_r, dist, cfg.vent_v));
+    });
+    z -= cfg.vent_sml;
+  }
+}
+
 }  // namespace psph
 
 using namespace psph;
+
+#define PSPH_TRY_H(expr)          \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_) return rc_;          \
+  } while (0)
 
 // ===========================================================================
 // C ABI: configuration
@@ -506,7 +583,7 @@ extern "C" int psph_meteo_standard(double t0, double p0, double z_top_km, double
 }
 
 // Meteo::interpolate (meteo.cc:64-143)
-static double meteo_interp(const psph_meteo *m, double z0, int prop) {
+double psph::meteo_interp(const psph_meteo *m, double z0, int prop) {
   const double *z = m->z, *d = m->data + (size_t)prop * m->rows;
   int R = m->rows;
   for (int r = 0; r < R; r++)
@@ -540,7 +617,7 @@ extern "C" int psph_gen_plume(const psph_config *cfg, const psph_meteo *met, uin
   if (!cfg || !met || !n_out) return fail(PSPH_EINVAL, "null argument");
   if (cfg->dim != 3) return fail(PSPH_EINVAL, "psph_gen_plume needs dim 3");
   if (!(cfg->bucket_edge > 0) || !(cfg->vent_sml > 0)) return fail(PSPH_EINVAL, "call psph_config_domain and psph_config_vent first");
-  const double dx = cfg->h_air, dx2 = 0.5 * dx, del = cfg->bucket_edge;
+  const double dx = cfg->h_air, del = cfg->bucket_edge;
   const int per = (int)std::llround(del / dx);
   int64_t nl[3];
   for (int k = 0; k < 3; k++) nl[k] = std::llround((cfg->ext_hi[k] - cfg->ext_lo[k]) / del) * per;
@@ -552,71 +629,25 @@ extern "C" int psph_gen_plume(const psph_config *cfg, const psph_meteo *met, uin
   if (!out) { *n_out = bound; return PSPH_OK; }
 
   HostParticles P;
-  P.reserve(bound);
-  const double rvsq = cfg->vent_r * cfg->vent_r;
-  for (int64_t lz = 0; lz < nl[2]; lz++)
-    for (int64_t ly = 0; ly < nl[1]; ly++)
-      for (int64_t lx = 0; lx < nl[0]; lx++) {
-        int64_t l[3] = {lx, ly, lz};
-        double pc[3];
-        for (int k = 0; k < 3; k++) {
-          double mincrd = cfg->ext_lo[k] + (double)(l[k] / per) * del;   // preprocess.cc:147-159
-          pc[k] = mincrd + dx2 + (double)(l[k] % per) * dx;               // add_air.cc:575-577
+  if (cfg->adapt_domain) {
+    // domain adaptation: the initial domain's buckets and their ghost layers
+    PSPH_TRY_H(gen_plume_adaptive(*cfg, met, seed, jitter, P));
+  } else {
+    P.reserve(bound);
+    for (int64_t lz = 0; lz < nl[2]; lz++)
+      for (int64_t ly = 0; ly < nl[1]; ly++)
+        for (int64_t lx = 0; lx < nl[0]; lx++) {
+          const int64_t l[3] = {lx, ly, lz};
+          double exact[3], pc[3];
+          lattice_point(*cfg, seed, jitter, l, nl, exact, pc);
+          int bc;
+          if (!(pc[2] > cfg->dom_lo[2])) bc = PSPH_BC_WALL_GHOST;
+          else if (pc[0] < cfg->dom_lo[0] || pc[0] > cfg->dom_hi[0] || pc[1] < cfg->dom_lo[1] ||
+                   pc[1] > cfg->dom_hi[1] || pc[2] > cfg->dom_hi[2]) bc = PSPH_BC_PRESSURE_GHOST;
+          else bc = PSPH_BC_REAL;
+          push_air(*cfg, met, P, pc, cfg->ratio_sml_dx * cfg->h_air, bc, bc == PSPH_BC_REAL ? 1 : 0, 0);
         }
-        if (jitter > 0) {
-          uint64_t L = (uint64_t)((lz * nl[1] + ly) * nl[0] + lx);
-          uint32_t r[4] = {uint32_t(L), uint32_t(L >> 32), 0, 0};
-          philox(r, seed, 0x50535048u);
-          for (int k = 0; k < 3; k++) {
-            double u = ((double)r[k] + 0.5) * (1.0 / 4294967296.0);
-            pc[k] += (2.0 * u - 1.0) * jitter * dx;
-          }
-        }
-        int bc;
-        if (!(pc[2] > cfg->dom_lo[2])) bc = PSPH_BC_WALL_GHOST;
-        else if (pc[0] < cfg->dom_lo[0] || pc[0] > cfg->dom_hi[0] || pc[1] < cfg->dom_lo[1] ||
-                 pc[1] > cfg->dom_hi[1] || pc[2] > cfg->dom_hi[2]) bc = PSPH_BC_PRESSURE_GHOST;
-        else bc = PSPH_BC_REAL;
-        // setup_erupt removes every non-erupt particle inside the vent duct
-        // (setup_erupt.cc:302-320)
-        {
-          double dist = 0;
-          dist += pc[0] * pc[0];
-          dist += pc[1] * pc[1];
-          if (dist <= rvsq && pc[2] <= cfg->vent_z && pc[2] >= cfg->vent_bot) continue;
-        }
-        size_t i = P.push();
-        P.x[i] = pc[0]; P.y[i] = pc[1]; P.z[i] = pc[2];
-        double normc[3];
-        for (int k = 0; k < 3; k++) normc[k] = (pc[k] - cfg->ext_lo[k]) / (cfg->ext_hi[k] - cfg->ext_lo[k]);
-        uint32_t key[3];
-        thsfc3d(normc, 0, key);
-        P.k0[i] = key[0]; P.k1[i] = key[1]; P.k2[i] = key[2];
-        P.h[i] = cfg->ratio_sml_dx * cfg->h_air;
-        P.bc[i] = bc; P.phase[i] = 1; P.inv[i] = bc == PSPH_BC_REAL ? 1 : 0; P.xi[i] = 0;
-        // initial_air + air_prop_meteo_based (sph_lib.cc:724-766, 831-872)
-        double sml2 = 0.5 * P.h[i];
-        double range[6] = {pc[0] - sml2, pc[0] + sml2, pc[1] - sml2, pc[1] + sml2, pc[2] - sml2, pc[2] + sml2};
-        double pressure = meteo_interp(met, pc[2] * 0.001, 1) * 100;
-        double density = meteo_interp(met, pc[2] * 0.001, 0);
-        double energy = pressure / (density * (cfg->gamma_p - 1));
-        double d1 = meteo_interp(met, range[4] * 0.001, 0);
-        double d2 = meteo_interp(met, range[5] * 0.001, 0);
-        double mass = (range[1] - range[0]) * (range[3] - range[2]) * (range[5] - range[4]) *
-                      (0.1666667 * d1 + 0.6666666 * density + 0.1666667 * d2);
-        P.rho[i] = density; P.e[i] = energy; P.p[i] = pressure; P.m[i] = mass;
-        eos(*cfg, density, energy, 0.0, P.p[i], P.c[i], P.T[i], P.gam[i], P.cp[i]);
-      }
-  // vent column (setup_erupt.cc:105-205)
-  double z = cfg->vent_z - 0.5 * cfg->vent_sml;
-  while (z >= cfg->vent_bot) {
-    vent_columns(*cfg, [&](double x, double y) {
-      double dist = 0;
-      dist += x * x;
-      dist += y * y;
-      if (dist <= rvsq) new_erupt(*cfg, P, x, y, z, 0, vel_prof(*cfg, cfg->vent_r, dist, cfg->vent_v));
-    });
-    z -= cfg->vent_sml;
+    setup_vent(*cfg, P);
   }
   auto perm = canonical_order(*cfg, P.k0.data(), P.k1.data(), P.k2.data(), P.size());
   P.permute(perm);

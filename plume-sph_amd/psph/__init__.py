@@ -20,10 +20,11 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpsph.so")
+# PSPH_LIB names another build of the library (a variant under study)
+LIB_PATH = os.environ.get("PSPH_LIB") or os.path.join(_HERE, "libpsph.so")
 
-PSPH_FP64, PSPH_FP32_PAIR, PSPH_FP32_PURE = 0, 1, 2
-PSPH_KERNEL_GAUSS_BOX, PSPH_KERNEL_CUBIC_M4 = 0, 1
+PSPH_FP64, PSPH_FP32_PAIR, PSPH_FP32_PURE, PSPH_FP32_EXP = 0, 1, 2, 3
+PSPH_KERNEL_GAUSS_BOX = 0
 BC_ERUPT_GHOST, BC_PRESSURE_GHOST, BC_WALL_GHOST, BC_NOSAVE_REAL, BC_REAL = 0, 1, 2, 99, 100
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EINVARIANT", -4: "ECAPACITY",
@@ -50,11 +51,12 @@ _DBL_TAIL = ["t_end", "t_output", "erupt_start", "erupt_end", "vent_r", "vent_v"
 
 class Config(C.Structure):
     """psph_config (include/psph.h)."""
-    _fields_ = ([(f, C.c_int32) for f in _INT_FIELDS] + [("reserved_i", C.c_int32 * 1)]
+    _fields_ = ([(f, C.c_int32) for f in _INT_FIELDS] + [("adapt_domain", C.c_int32)]
                 + [(f, C.c_double) for f in _DBL_FIELDS]
                 + [("dom_lo", C.c_double * 3), ("dom_hi", C.c_double * 3),
                    ("ext_lo", C.c_double * 3), ("ext_hi", C.c_double * 3)]
-                + [(f, C.c_double) for f in _DBL_TAIL] + [("reserved_d", C.c_double * 8)])
+                + [(f, C.c_double) for f in _DBL_TAIL]
+                + [("idom_lo", C.c_double * 3), ("idom_hi", C.c_double * 3), ("reserved_d", C.c_double * 2)])
 
     def copy(self) -> "Config":
         c = Config()
@@ -82,7 +84,7 @@ class StepOut(C.Structure):
     _fields_ = [("dt", C.c_double), ("time", C.c_double), ("step", C.c_int64),
                 ("n_total", C.c_int64), ("n_real", C.c_int64), ("n_injected", C.c_int64),
                 ("n_removed", C.c_int64), ("n_pairs", C.c_int64),
-                ("output", C.c_int64)]
+                ("output", C.c_int64), ("n_domain", C.c_int64)]
 
 
 class AdjointOut(C.Structure):
@@ -152,6 +154,14 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise PsphError(-7, f"{LIB_PATH} is not built: run plume-sph_amd/build.sh")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same
+    # soname, another file).  Loaded first, it is the one libpsph binds to;
+    # loaded after libpsph had pulled in the system's copy, a second runtime
+    # comes up that finds no GPU.  So torch, where installed, is loaded first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P, CFG, SOA = C.c_void_p, C.POINTER(Config), C.POINTER(SoA)
     sig = {
@@ -171,6 +181,7 @@ def lib() -> C.CDLL:
         "psph_count": (C.c_int, [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
         "psph_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
         "psph_get_config": (C.c_int, [P, CFG]),
+        "psph_device_view": (C.c_int, [P, SOA, C.POINTER(C.c_size_t), C.POINTER(C.c_void_p)]),
         "psph_timestep": (C.c_int, [P, C.POINTER(C.c_double)]),
         "psph_advance_time": (C.c_int, [P, C.POINTER(C.c_double)]),
         "psph_neighbors": (C.c_int, [P]),
@@ -182,6 +193,7 @@ def lib() -> C.CDLL:
         "psph_inject": (C.c_int, [P, C.c_double, C.POINTER(C.c_int64)]),
         "psph_apply_bcond": (C.c_int, [P]),
         "psph_step": (C.c_int, [P, C.POINTER(StepOut)]),
+        "psph_domain_init": (C.c_int, [P, C.POINTER(Meteo), C.c_uint32, C.c_double]),
         "psph_run": (C.c_int, [P, C.c_int64, C.POINTER(StepOut)]),
         "psph_get_neighbors": (C.c_int, [P, C.POINTER(C.c_int64), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t]),
         "psph_conservation": (C.c_int, [P, C.POINTER(C.c_double)]),
@@ -348,6 +360,26 @@ class Solver:
         check(lib().psph_read(self._ctx, C.byref(s), n.value, C.byref(n)))
         return P
 
+    def device_view(self) -> dict:
+        """Zero-copy torch tensors (HIP device) over the context's particle
+        arrays, keyed like Particles, in the context's current order
+        (psph_device_view).  They stay valid until the next load, sort
+        (neighbour search, step, run) or adjoint run.  Work that torch queues
+        on them must be finished (torch.cuda.synchronize) before the next
+        library call, which runs on the library's own stream."""
+        import torch
+        from .dlpack import KEEP, as_tensor
+        s, n, st = SoA(), C.c_size_t(), C.c_void_p()
+        check(lib().psph_device_view(self._ctx, C.byref(s), C.byref(n), C.byref(st)))
+        dev = torch.cuda.current_device()
+        keep = KEEP
+        out = {}
+        for k in ALL_FIELDS:
+            ptr = C.cast(getattr(s, k), C.c_void_p).value
+            dt = torch.float64 if k in D_FIELDS else (torch.uint32 if k in U_FIELDS else torch.int32)
+            out[k] = as_tensor(ptr, n.value, dt, dev, keep)
+        return out
+
     def count(self):
         a, b = C.c_size_t(), C.c_size_t()
         check(lib().psph_count(self._ctx, C.byref(a), C.byref(b)))
@@ -394,6 +426,11 @@ class Solver:
 
     def apply_bcond(self):
         check(lib().psph_apply_bcond(self._ctx))
+
+    def domain_init(self, met, seed: int, jitter: float = 0.01):
+        """Domain adaptation set-up after load (psph_domain_init): the
+        atmosphere, seed and jitter of the generated air lattice."""
+        check(lib().psph_domain_init(self._ctx, C.byref(met.struct()), seed, jitter))
 
     def step(self) -> StepOut:
         o = StepOut()

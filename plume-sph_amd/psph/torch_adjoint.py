@@ -7,7 +7,8 @@ initial-velocity perturbation ``dv0`` (n, 3) to the velocity v, runs
 height of erupted material).  Its backward is the hand-written adjoint:
 dH/d(vent_v, vent_T, mass_frac) and dH/d dv0 = dH/dv of the starting state.
 The forward computes the gradients eagerly (one reverse sweep per call);
-torch only routes them.
+torch only routes them.  A ``dv0`` on the GPU is applied in place through the
+library's zero-copy device view (psph_device_view), not through host memory.
 
 The smoothed velocity sv is the velocity filter's state and stays as
 generated.  (In a fluid at rest H is not differentiable in sv: the heat
@@ -41,10 +42,22 @@ def build_state(cfg: psph.Config, met, seed: int, jitter: float, vent_v: float, 
 class PlumeHeight(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vent_v, vent_T, mass_frac, dv0, cfg, met, seed, jitter, nsteps, ckpt_every, extra_capacity):
+        on_device = dv0.is_cuda
         c, P = build_state(cfg, met, seed, jitter, vent_v.item(), vent_T.item(), mass_frac.item(),
-                           dv0.detach().cpu().numpy())
+                           None if on_device else dv0.detach().cpu().numpy())
         with psph.Solver(c, len(P) + extra_capacity) as S:
             S.load(P)
+            if on_device:
+                # dv0 stays on the device: added in place through the zero-copy
+                # view, whose order right after the load is P's
+                if dv0.shape != (len(P), 3):
+                    raise ValueError(f"dv0 has shape {tuple(dv0.shape)}, the state {len(P)} particles")
+                V = S.device_view()
+                d = dv0.detach().to(torch.float64)
+                for k, f in enumerate(("vx", "vy", "vz")):
+                    V[f].add_(d[:, k])
+                torch.cuda.synchronize()
+                del V
             r = S.adjoint_run(nsteps, ckpt_every, grads=True)
         if r["n_particles"] != len(P):
             raise psph.PsphError(-6, f"adjoint returned {r['n_particles']} gradients for {len(P)} particles")
