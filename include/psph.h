@@ -304,6 +304,20 @@ int psph_conservation(psph_ctx *ctx, double out[7]);
 /* Plume diagnostic: mass-weighted mean height of erupted material over real
  * particles, H = sum m xi z / sum m xi.  No reference counterpart. */
 int psph_diag_height(psph_ctx *ctx, double *H);
+/* Allocation-free variant of psph_conservation + psph_diag_height, with no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It needs no context: dev holds DEVICE pointers of n
+ * particles (bc, m, e, vx, xi; also vy, vz, z when dim is 3; e.g. from
+ * psph_device_view), out_dev is a DEVICE array of 8 doubles that receives the
+ * 7 sums of psph_conservation and sum m xi z (H = out[7] / out[6]), and
+ * stream is a hipStream_t (NULL: the null stream).  The work is queued on the
+ * stream and not waited for.  It allocates nothing: its block partials live in
+ * a 32 KiB array in the library's device memory, one per GPU, held until the
+ * process ends, so calls that overlap on one GPU race.  n above 4294967040
+ * (2^32 - 256) is rejected with PSPH_EINVAL.  Results are bit-identical to
+ * psph_conservation's on the same particles in the same order. */
+int psph_conservation_noalloc(const psph_soa *dev, size_t n, int32_t dim, double *out_dev, void *stream);
 
 /* ---- adjoint ------------------------------------------------------------ */
 /* Reverse-mode gradient of the plume diagnostic H (psph_diag_height) after

@@ -2991,3 +2991,4 @@ extern "C" int psph_profile_read(psph_ctx *c, double ms[PSPH_NSTAGES], int64_t l
 
 #include "psph_adjoint.inl"
 #include "psph_domain.inl"
+#include "psph_noalloc.inl"

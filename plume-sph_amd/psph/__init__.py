@@ -198,6 +198,7 @@ def lib() -> C.CDLL:
         "psph_get_neighbors": (C.c_int, [P, C.POINTER(C.c_int64), C.c_size_t, C.POINTER(C.c_uint32), C.c_size_t]),
         "psph_conservation": (C.c_int, [P, C.POINTER(C.c_double)]),
         "psph_diag_height": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "psph_conservation_noalloc": (C.c_int, [SOA, C.c_size_t, C.c_int32, C.c_void_p, C.c_void_p]),
         "psph_last_error": (C.c_char_p, []),
         "psph_version": (C.c_char_p, []),
         "psph_abi_sizeof": (C.c_size_t, [C.c_int]),
