@@ -56,8 +56,8 @@ extern "C" {
                                 FP64 over the same run                         */
 #define PSPH_FP32_EXP    3   /* PSPH_FP32_PAIR with the pair weights'
                                 exponential in FP32 hardware (argument formed
-                                in FP64, its rounding taken back out; h, 1/h,
-                                norms and sums FP64): ~1 FP32 ulp per weight  */
+                                in FP64 and rounded once; h, 1/h, norms and
+                                sums FP64): about an FP32 ulp per weight      */
 
 /* ---- smoothing kernel -------------------------------------------------- */
 /* The reference has one kernel, a Gaussian truncated per axis at |s_k| = 3
@@ -332,8 +332,12 @@ int psph_conservation_noalloc(const psph_soa *dev, size_t n, int32_t dim, double
  * kernel boxes, the list and support boxes; a box indicator has zero
  * derivative wherever it is differentiable).  The recorded run keeps every
  * step's particle positions by id (24 B per particle and step) for
- * psph_replay_run.  Erupt ghosts present at the start (bc 0) are taken to hold
- * the vent state of the config, as psph_gen_plume makes them.
+ * psph_replay_run.  With cfg.wall_bc the wall-ghost refresh that ends every
+ * step is differentiated too (its contributor sets frozen like the pair set;
+ * the step records take 84 B and the recorded run 24 B more per particle and
+ * step); a context with cfg.adapt_domain is refused (PSPH_ENOTSUP).  Erupt
+ * ghosts present at the start (bc 0) are taken to hold the vent state of the
+ * config, as psph_gen_plume makes them.
  * The forward run stores a checkpoint every ckpt_every steps; the reverse
  * sweep recomputes one segment at a time from its checkpoint.  grad_v and
  * grad_sv (either may be NULL) receive dH/dv and dH/dsv of the starting
@@ -356,9 +360,10 @@ int psph_adjoint_run(psph_ctx *ctx, int64_t nsteps, int64_t ckpt_every, psph_adj
 /* Runs nsteps steps with the frozen schedule of the last psph_adjoint_run
  * (the map the adjoint differentiates; finite-difference checks use it):
  * besides the schedule, the cell order, tiles, visit lists and every pair
- * predicate of a step are evaluated on the recorded run's positions of that
- * step, and only the pair arithmetic on the replay's own, so the replay
- * visits the recorded pairs in the recorded summation order.  The context
+ * predicate of a step (and, with cfg.wall_bc, the wall refresh's support
+ * tests) are evaluated on the recorded run's positions of that step, and
+ * only the arithmetic on the replay's own, so the replay visits the recorded
+ * pairs in the recorded summation order.  The context
  * must hold a state loaded at that run's start time and step, with the same
  * particle count. */
 int psph_replay_run(psph_ctx *ctx, int64_t nsteps);

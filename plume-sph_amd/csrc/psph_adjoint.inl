@@ -170,6 +170,7 @@ struct AdjCommon {
   const int32_t *bc;
   Traversal tr;
   int list_all;
+  int list_adj;              // some list box may reach past the adjacent buckets
   double list_factor, cutoff2;
   CutTest cut;
 };
@@ -290,12 +291,13 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
             const AdjVisitRow R = rw[t];
             const double amax = fmax(fabs(R.x - cxp), fmax(fabs(R.y - cyp), fabs(R.z - czp)));
             const bool same = (pc.list_all != 0) | (crb == R.refb);
+            const bool adj = (pc.list_adj == 0) || refb_adjacent(crb, R.refb);
             const double hij = 0.5 * (R.h + hj);
             // row t sums, this lane's candidate is its neighbour
-            const bool own = (R.tgt != 0) & (same | (amax <= R.lf)) & (amax <= R.supp) &
+            const bool own = (R.tgt != 0) & (same | ((amax <= R.lf) & adj)) & (amax <= R.supp) &
                              (amax <= cutx * fmax(R.h, hij));
             // this lane's candidate sums, row t is its neighbour
-            const bool nbr = nn_j & (same | (amax <= lfj)) & (amax <= c2hj) & (amax <= cutx * fmax(hj, hij));
+            const bool nbr = nn_j & (same | ((amax <= lfj) & adj)) & (amax <= c2hj) & (amax <= cutx * fmax(hj, hij));
             const uint64_t bo = __ballot(own), bn = __ballot(nbr);
             // lanes 2t and 2t+1 (row t's sub-lanes) take the two halves
             const uint64_t ho = odd ? (bo >> 32) : (bo & 0xffffffffull);
@@ -959,6 +961,7 @@ static int rec_capture(psph_ctx *c, int stage, double dt) {
     r.n = n;
     r.grid = c->grid;
     r.reach = c->reach_main;
+    r.hmax = c->h_max;
     r.dt = dt;
     r.time = c->time;
     const double *src[] = {c->f.x, c->f.y, c->f.z, c->f.h, c->f.m, c->f.rho, c->f.p, c->f.c, c->f.T, c->f.cp,
@@ -1144,8 +1147,9 @@ struct ThetaGrad { double v = 0, T = 0, ng0 = 0; };
 // the adjoints of the ghost's values before the refresh restart at zero.
 // Contributors gather their ghosts (no atomics): a contributor's candidates
 // are the wall ghosts around its mirror point, found in the step's cell grid
-// (wall ghosts do not move).  Pressure ghosts contribute to the refresh but
-// hold constant states, so their adjoints are not formed.
+// (wall ghosts do not move).  Pressure ghosts contribute as well: their
+// state is never updated, so the adjoint of their initial state gathers the
+// refresh's terms of every step.
 enum { BA_ARHO, BA_AVX, BA_AVY, BA_AVZ, BA_AE, BA_AXI, BA_B, BA_GNG0, BA_N };
 struct BcondAdj { double *f[BA_N]; };
 
@@ -1203,7 +1207,7 @@ __global__ void k_adj_bcond_contr(uint32_t n, BcondArgs g, BcondAdjIn in, BcondA
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int b = in.bcn[j];
-  if (b != 100 && b != 99) return;
+  if (b != 100 && b != 99 && b != 1) return;        // contr_image (particle.h:562-565)
   const double pj[3] = {in.xn[j], in.yn[j], in.zn[j]};
   int bj[3];
   if (!bucket_of(g, pj, bj)) return;
@@ -1352,6 +1356,7 @@ static int reverse_step(psph_ctx *c, Adjoint &A, StepRec &R, ThetaGrad &th) {
   pc.bc = R.bc;
   pc.tr = Traversal{R.cs, R.toff, R.tbox, R.grid, R.reach};
   pc.list_all = k.list_all ? 1 : 0;
+  pc.list_adj = list_adj(k, R.hmax);
   pc.list_factor = k.list_factor; pc.cutoff2 = k.cutoff2;
   pc.cut = cut_test(k.cutoff);
   AdjVisits vl;

@@ -244,8 +244,22 @@ __global__ __launch_bounds__(kWave) void k_tile_boxes(uint32_t ncells, const uin
   }
 }
 
-// reference bucket id (preprocessor cells of edge 8 h_air, preprocess.cc:88-133)
-// and the smoothing length at list time
+// Reference bucket (preprocessor cells of edge 8 h_air, preprocess.cc:88-133)
+// as packed coordinates: x in bits 0-9, y in bits 11-20, z in bits 22-31 (at
+// most 1023 buckets per axis; kDead outside the grid).  A list holds the own
+// bucket's particles and, from the 26 adjacent buckets, those within the
+// list box (neighs.cc:37-211).
+constexpr int kRefbMaxPerAxis = 1023;
+__device__ inline uint32_t refb_pack(uint32_t x, uint32_t y, uint32_t z) { return x | (y << 11) | (z << 22); }
+// buckets at most one apart on every axis: per field 1024 + a - b + 1 (the
+// guard bits 10 and 21 stop borrows between fields), (a - b + 1) mod 1024 for z
+__device__ inline bool refb_adjacent(uint32_t a, uint32_t b) {
+  if (a == kDead || b == kDead) return false;
+  const uint32_t d = ((a | 0x00200400u) - b) + 0x00400801u;
+  return ((d & 0x7ffu) - 1024u <= 2u) & (((d >> 11) & 0x7ffu) - 1024u <= 2u) & ((d >> 22) <= 2u);
+}
+
+// the smoothing length at list time, and the reference bucket
 __global__ void k_refb(uint32_t n, const double *x, const double *y, const double *z, const double *h,
                        double lx, double ly, double lz, double del, int bx, int by, int bz,
                        uint32_t *refb, double *h_list) {
@@ -254,7 +268,7 @@ __global__ void k_refb(uint32_t n, const double *x, const double *y, const doubl
   double fx = floor((x[i] - lx) / del), fy = floor((y[i] - ly) / del), fz = floor((z[i] - lz) / del);
   uint32_t r = kDead;
   if (fx >= 0 && fx < bx && fy >= 0 && fy < by && fz >= 0 && fz < bz)
-    r = ((uint32_t)fz * by + (uint32_t)fy) * bx + (uint32_t)fx;
+    r = refb_pack((uint32_t)fx, (uint32_t)fy, (uint32_t)fz);
   refb[i] = r;
   h_list[i] = h[i];
 }
@@ -349,10 +363,47 @@ struct PairCommon {
   const uint32_t *targets;
   const uint32_t *cfirst, *ccount, *nchunks;
   int list_all;
+  int list_adj;               // some list box may reach past the adjacent buckets
   double list_factor;
+  double del;                 // reference bucket edge
+  double ext_lo[3];           // origin of the reference buckets
   CutTest cut;
   int *err;
 };
+
+// The region a chunk's neighbour lists can reach (neighs.cc:37-211): per
+// target its own reference bucket and, within the adjacent buckets, its list
+// box; the chunk's box is their union, with a margin for the rounding of the
+// bucket index (1e-9 of a bucket).  It is often much smaller than the
+// traversal's reach, which has to bound the largest list box of all.
+__device__ inline ChunkBox chunk_list_box(const PairCommon &pc, const double *p, double lf, uint32_t rb) {
+  ChunkBox b;
+  const int nn[3] = {pc.tr.g.nx, pc.tr.g.ny, pc.tr.g.nz};
+  const uint32_t bk[3] = {rb & 1023u, (rb >> 11) & 1023u, rb >> 22};
+  const double m = 1e-9 * pc.del;
+  for (int d = 0; d < 3; d++) {
+    double lo, hi;
+    if (rb == kDead || pc.list_all) {
+      lo = p[d] - pc.tr.reach;
+      hi = p[d] + pc.tr.reach;
+    } else {
+      const double blo = pc.ext_lo[d] + bk[d] * pc.del, bhi = blo + pc.del;
+      lo = fmin(fmax(p[d] - lf, blo - pc.del), blo) - m;
+      hi = fmax(fmin(p[d] + lf, bhi + pc.del), bhi) + m;
+    }
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    b.lo[d] = __double2float_rd(lo);
+    b.hi[d] = __double2float_ru(hi);
+    if (pc.tr.g.inv_e > 0) {
+      b.c0[d] = cell_of(lo, pc.tr.g.lo[d], pc.tr.g.inv_e, nn[d]);
+      b.c1[d] = cell_of(hi, pc.tr.g.lo[d], pc.tr.g.inv_e, nn[d]);
+    } else {
+      b.c0[d] = b.c1[d] = 0;
+    }
+  }
+  return b;
+}
 
 // Work split of a wavefront: kChunk targets (consecutive in the cell's fine
 // Morton order, so they sit within ~2 particle spacings of each other), each
@@ -397,7 +448,15 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
   const double lf_i = pc.list_factor * pc.h_list[i];
   typename P::Tgt t;
   P::load_target(a, i, t);
-  const ChunkBox box = chunk_box(pc.tr, xi);
+  // a chunk none of whose targets wants the pass (converged in an earlier
+  // iteration of the adaptive h) skips the traversal
+  if (!__any(active && P::want(t))) {
+    if (active && sub == 0) P::finish(a, t, i, pc.err);
+    return;
+  }
+  // the passes run here (adaptive h, list materialisation) accept list
+  // members only: the chunk visits the region its lists can reach
+  const ChunkBox box = chunk_list_box(pc, xi, lf_i, rb_i);
   const GridDesc &g = pc.tr.g;
   P::init(tile, lane);
   __syncthreads();
@@ -440,7 +499,8 @@ __global__ __launch_bounds__(kWave) void k_pairs(PairCommon pc, typename P::Args
                 // (neighs.cc:37-211: same bucket or 4.25 h_i box), in_support
                 // and the pass's own mask
                 bool in = !pc.cut.outside(amax, P::hker(a, t, tile, q));
-                in &= (pc.list_all != 0) | (srefb[q] == rb_i) | (amax <= lf_i);
+                in &= (pc.list_all != 0) | (srefb[q] == rb_i) |
+                      ((amax <= lf_i) & ((pc.list_adj == 0) || refb_adjacent(srefb[q], rb_i)));
                 in &= (q < cnt) & (amax <= t.supp) & P::extra(a, t, tile, q, base + q, i);
                 mask |= (Mask)in << b;
               }
@@ -507,7 +567,30 @@ __device__ inline uint32_t count_meeting_tiles(const Traversal &tr, const ChunkB
   return cnt;
 }
 
-__global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *vcount) {
+// The chunk's grown box for the visit lists: per target the widest kernel
+// box of the three passes, cutx max((h_i + h_max) / 2, h_i), with a relative
+// margin of 1e-9 (the traversal's reach bounds it with h_i = h_max).
+__device__ inline ChunkBox chunk_pass_box(const Traversal &tr, const double *p, double h, double cutx,
+                                          double h_max) {
+  ChunkBox b;
+  const int nn[3] = {tr.g.nx, tr.g.ny, tr.g.nz};
+  const double r = cutx * fmax(0.5 * h + 0.5 * h_max, h) * (1 + 1e-9);
+  for (int d = 0; d < 3; d++) {
+    const double lo = wave_min(p[d] - r), hi = wave_max(p[d] + r);
+    b.lo[d] = __double2float_rd(lo);
+    b.hi[d] = __double2float_ru(hi);
+    if (tr.g.inv_e > 0) {
+      b.c0[d] = cell_of(lo, tr.g.lo[d], tr.g.inv_e, nn[d]);
+      b.c1[d] = cell_of(hi, tr.g.lo[d], tr.g.inv_e, nn[d]);
+    } else {
+      b.c0[d] = b.c1[d] = 0;
+    }
+  }
+  return b;
+}
+
+__global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, const double *h, double cutx, double h_max,
+                                                       uint32_t *vcount) {
   const uint32_t chunk = blockIdx.x;
   const uint32_t nch = *pc.nchunks;
   if (chunk >= nch) {
@@ -518,7 +601,7 @@ __global__ __launch_bounds__(kWave) void k_visit_count(PairCommon pc, uint32_t *
   const uint32_t slot = (uint32_t)threadIdx.x / kSub;
   const uint32_t i = pc.targets[cf + (slot < cc ? slot : 0)];
   const double xi[3] = {pc.x[i], pc.y[i], pc.z[i]};
-  const ChunkBox box = chunk_box(pc.tr, xi);
+  const ChunkBox box = chunk_pass_box(pc.tr, xi, h[i], cutx, h_max);
   const uint32_t cnt = count_meeting_tiles(pc.tr, box);
   if (threadIdx.x == 0) vcount[chunk] = cnt;
 }
@@ -536,8 +619,9 @@ struct VisitTgt {
 };
 
 // FAST: for every target of the chunk the widened box lies inside both its
-// list box (4.25 h at list time) and its in_support box (CUTOFF2 h), so those
-// two tests cannot fail for a candidate inside the widened box.
+// list box (4.25 h at list time) and its in_support box (CUTOFF2 h), and is
+// narrower than a bucket (so it reaches the adjacent buckets only): those
+// tests cannot fail for a candidate inside the widened box.
 template <int D>
 __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const double *h, double cutoff2, double cutx,
                                                        double h_max, const uint32_t *voff, uint32_t *vnum,
@@ -569,17 +653,18 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
     blo[d] = D == 3 || d == 0 ? __double2float_rd(xi[d] - widest) : -INFINITY;
     bhi[d] = D == 3 || d == 0 ? __double2float_ru(xi[d] + widest) : INFINITY;
   }
-  const bool fast = __all(!owner || (widest <= supp_i && (pc.list_all != 0 || widest <= lf_i)));
+  const bool fast = __all(!owner || (widest <= supp_i &&
+                                     (pc.list_all != 0 || (widest <= lf_i && widest * (1 + 1e-12) < pc.del))));
   // the chunk's grown box (same as the passes': every lane's target counts)
   const uint32_t slot = (uint32_t)lane / kSub;
   const uint32_t ib = pc.targets[cf + (slot < cc ? slot : 0u)];
   const double xb[3] = {pc.x[ib], pc.y[ib], pc.z[ib]};
-  const ChunkBox box = chunk_box(pc.tr, xb);
+  const ChunkBox box = chunk_pass_box(pc.tr, xb, h[ib], cutx, h_max);
   __syncthreads();
   const GridDesc &g = pc.tr.g;
   const uint32_t cq = 2u * ((uint32_t)lane & 31u) + ((uint32_t)lane >> 5);   // this lane's candidate
   const uint32_t mine = (uint32_t)lane >> 1;                                 // this lane's target slot
-  const bool odd = (lane & 1) != 0;
+  const uint32_t hsh = 32u * ((uint32_t)lane & 1u);                          // its half of a ballot
   const uint32_t w0 = voff[chunk];
   uint32_t w = w0;
   for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
@@ -605,24 +690,35 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
           const double chh = valid ? cutx * (0.5 * h[j]) : 0.0;
           const uint32_t crb = valid ? pc.refb[j] : ~0u;
           uint32_t mask = 0;
-          while (tm) {
-            const uint32_t t = (uint32_t)__builtin_ctzll(tm);
-            tm &= tm - 1;
-            const VisitTgt T = tg[t];
-            double amax = fabs(T.x - cxp);
-            if (D == 3) amax = fmax(amax, fmax(fabs(T.y - cyp), fabs(T.z - czp)));
-            // amax <= cutx max(h_ij, h_t), with cutx h_ij ~ cutx hh_t + cutx hh_j
-            // (the 1e-12 margin of cutx covers the rounding of the split sum)
-            bool in = (amax <= T.chh + chh) | (amax <= T.ch);
-            if (!fast) {
-              in &= (pc.list_all != 0) | (crb == trb[t]) | (amax <= T.lf);
-              in &= amax <= T.supp;
+          // the same loop with and without the list and support tests (the
+          // wave-uniform choice taken outside it)
+          auto scan = [&](auto fast_c) {
+            while (tm) {
+              const uint32_t t = (uint32_t)__builtin_ctzll(tm);
+              tm &= tm - 1;
+              const VisitTgt T = tg[t];
+              double amax = fabs(T.x - cxp);
+              if (D == 3) amax = fmax(amax, fmax(fabs(T.y - cyp), fabs(T.z - czp)));
+              // amax <= cutx max(h_ij, h_t), with cutx h_ij ~ cutx hh_t + cutx hh_j
+              // (the 1e-12 margin of cutx covers the rounding of the split sum,
+              // taken as amax - cutx hh_j <= cutx hh_t)
+              uint64_t bal;
+              if constexpr (decltype(fast_c)::value) {
+                // one ballot per compare: each compare writes its lane mask directly
+                bal = __ballot(amax - chh <= T.chh) | __ballot(amax <= T.ch);
+              } else {
+                bool in = (amax - chh <= T.chh) | (amax <= T.ch);
+                in &= (pc.list_all != 0) | (crb == trb[t]) |
+                      ((amax <= T.lf) & ((pc.list_adj == 0) || refb_adjacent(crb, trb[t])));
+                in &= amax <= T.supp;
+                bal = __ballot(in);
+              }
+              // lanes 2t and 2t+1 (target t's sub-lanes) take the two halves
+              const uint32_t half = (uint32_t)(bal >> hsh);
+              mask = mine == t ? half : mask;
             }
-            const uint64_t bal = __ballot(in);
-            // lanes 2t and 2t+1 (target t's sub-lanes) take the two halves
-            const uint32_t half = odd ? (uint32_t)(bal >> 32) : (uint32_t)bal;
-            mask = mine == t ? half : mask;
-          }
+          };
+          if (fast) scan(std::true_type{}); else scan(std::false_type{});
           if (__ballot(mask != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
             __builtin_nontemporal_store(mask, vmask + (size_t)w * kWave + lane);
@@ -775,25 +871,23 @@ __device__ inline double exp_neg_tab(double e, const double *tab) {
 
 // The exponential of the FP32-pair forms' FP64 weights.  FX false
 // (PSPH_FP32_PAIR): exp_neg_tab, FP64 throughout.  FX true (PSPH_FP32_EXP):
-// the FP32 hardware exponential 2^-tf of the argument t = e log2(e) formed in
-// FP64 and rounded once to tf, times 1 - (t - tf) ln 2, which takes the
-// rounding of t back out to second order: the relative error is the
-// instruction's own (about one FP32 ulp), independent from pair to pair,
-// while h, 1/h and the norm stay FP64.
+// the FP32 hardware exponential 2^-tf of the argument t = e log2(e), formed
+// in FP64 and rounded once to tf: a relative error of the instruction's own
+// (about one FP32 ulp) plus t 2^-24 ln 2 from the rounding of t (largest at
+// the kernel box's corner, where the weight is e^-27 of the centre's), while
+// h, 1/h and the norm stay FP64.  Taking the rounding of t back out (times
+// 1 - (t - tf) ln 2) halves the form's difference to FP64 over the bench's
+// steps and costs 4 % of the step: not kept.
 template <bool FX> constexpr int wtab_size() { return FX ? 1 : kETab; }
 template <bool FX> __device__ inline void wtab_fill(double *tab, int lane) {
   if constexpr (!FX) etab_fill(tab, lane);
 }
+constexpr double kLog2e = 1.4426950408889634;
+// 2^-t of the FX form (t >= 0 in FP64)
+__device__ inline double exp2_neg_fx(double t) { return (double)__builtin_amdgcn_exp2f(-(float)t); }
 template <bool FX> __device__ inline double exp_neg_w(double e, const double *tab) {
-  if constexpr (!FX) {
-    return exp_neg_tab(e, tab);
-  } else {
-    constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
-    const double t = e * kLog2e;
-    const float tf = (float)t;
-    const double E = (double)__builtin_amdgcn_exp2f(-tf);
-    return fma(-E * kLn2, t - (double)tf, E);
-  }
+  if constexpr (!FX) return exp_neg_tab(e, tab);
+  else return exp2_neg_fx(e * kLog2e);
 }
 
 template <class R> __device__ inline R pow16(R x) {
@@ -809,26 +903,39 @@ struct RhsArgs {
   const int32_t *bc;
   double *o_vx, *o_vy, *o_vz, *o_e, *o_p, *o_c, *o_T, *o_gam, *o_cp;
   double dt, g, cutoff, cutoff2, alpha, beta, ata, eps_half, prandtl, heat_scale;
+  // per-pair constants, folded on the host: 2 beta; 20 eps / Pr (the heat
+  // term's 10 eps / Pr, doubled because the energy sum holds 2 de/dt)
+  double beta2, kheat;
   int lans, dim;
   EosConst eos;
   unsigned long long *pairs;
 };
 
+// Pair terms, written with the halves folded: c2 = c_i + c_j = 2 c_ab,
+// rs = rho_i + rho_j = 2 rho_ab, cp2 = 2 cp_ab.  Pi = (beta mu - alpha c_ab)
+// mu / rho_ab = (2 beta mu - alpha c2) mu / rs with mu = h v.r / den (one
+// reciprocal of den rs gives both 1/den and 1/rs); the heat term's cp_ab /
+// c_ab = cp2 / c2.  The energy sum holds 2 de/dt (finish takes half).
+//
 // The FP32-pair form keeps the pressure term m_j (p_j/rho_j^2 + p_i/rho_i^2)
 // grad W in FP64: in air near hydrostatic balance its pair terms cancel to
 // about 1 % of their size, and FP32 rounding there is what the velocities
 // feel.  The kernel value comes from the exp table (FP64 at FP32-like cost);
 // viscosity, heat conduction and the energy work terms stay FP32.
-template <bool FX> struct RhsMixTile { double m64[kWave], pv64[kWave], etab[wtab_size<FX>()]; };
-struct NoTile {};
+template <bool FX> struct RhsMixTile { double m64[kWave], mpv64[kWave], etab[wtab_size<FX>()]; float mV[kWave]; };
+template <class R> struct RhsTileR { R PV2[kWave]; };
+
+// -2 pi^(-D/2): grad W = G dx with G = -2 W / h^2 = kGradNorm h^-(D+2) exp(-r^2/h^2)
+template <int D> constexpr double kGradNorm = -2.0 * kInvSqrtPi * kInvSqrtPi * kInvSqrtPi;
+template <> constexpr double kGradNorm<1> = -2.0 * kInvSqrtPi;
 
 template <class R, bool MIXED = std::is_same<R, float>::value, bool FX = false>
 struct RhsPass {
   using Args = RhsArgs;
   static constexpr bool kMixed = MIXED;
-  struct Tile : std::conditional<kMixed, RhsMixTile<FX>, NoTile>::type {
+  struct Tile : std::conditional<kMixed, RhsMixTile<FX>, RhsTileR<R>>::type {
     double hh[kWave];       // h / 2
-    R m[kWave], V[kWave], PV2[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
+    R m[kWave], V[kWave], rho[kWave], c[kWave], T[kWave], cp[kWave];
     R vx[kWave], vy[kWave], vz[kWave];
   };
   struct Tgt {
@@ -869,8 +976,10 @@ struct RhsPass {
     const double rj = a.rho[j];
     const double V = 1.0 / rj;
     const double pv = a.p[j] * V * V;
-    s.hh[l] = 0.5 * a.h[j]; s.m[l] = (R)a.m[j]; s.V[l] = (R)V; s.PV2[l] = (R)pv; s.rho[l] = (R)rj;
-    if constexpr (kMixed) { s.m64[l] = a.m[j]; s.pv64[l] = pv; }
+    const double mj = a.m[j];
+    s.hh[l] = 0.5 * a.h[j]; s.m[l] = (R)mj; s.V[l] = (R)V; s.rho[l] = (R)rj;
+    if constexpr (kMixed) { s.m64[l] = mj; s.mpv64[l] = mj * pv; s.mV[l] = (float)(mj * V); }
+    else s.PV2[l] = (R)pv;
     s.c[l] = (R)a.c[j]; s.T[l] = (R)a.T[j]; s.cp[l] = (R)a.cp[j];
     if (a.lans) { s.vx[l] = (R)a.svx[j]; s.vy[l] = (R)a.svy[j]; s.vz[l] = (R)a.svz[j]; }
     else { s.vx[l] = (R)a.vx[j]; s.vy[l] = (R)a.vy[j]; s.vz[l] = (R)a.vz[j]; }
@@ -882,30 +991,32 @@ struct RhsPass {
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *dxd, double) {
     t.npairs++;
+    if (!(s.rho[q] > R(0))) { t.bad = 1; return; }      // assert(uvecj[0] > 0)
+    if constexpr (kMixed) pair_mixed<D>(a, t, s, q, dxd);
+    else pair_plain<D>(a, t, s, q, dxd);
+  }
+  template <int D>
+  __device__ static void pair_plain(const Args &a, Tgt &t, const Tile &s, uint32_t q, const double *dxd) {
     const R rhoj = s.rho[q];
-    if (!(rhoj > R(0))) { t.bad = 1; return; }          // assert(uvecj[0] > 0)
-    if constexpr (kMixed) {
-      pair_mixed<D>(a, t, s, q, dxd);
-      return;
-    }
     const R h = (R)(t.hh + s.hh[q]);
     const R ih = rrcp<R>(h);
-    R dx[3], si[3], dist_sq = 0, expt = 0;
+    R dx[3], dist_sq = 0, expt = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
       dx[k] = (R)dxd[k];
-      si[k] = dx[k] * ih;
+      const R sk = dx[k] * ih;
       dist_sq += dx[k] * dx[k];
-      expt += si[k] * si[k];
+      expt += sk * sk;
     }
-    const R nk = R(kInvSqrtPi) * ih;
-    const R norm = (D == 3) ? nk * nk * nk : nk;
-    const R W = norm * rexp<R>(-expt);
+    const R ih2 = ih * ih;
+    const R ihD = (D == 3) ? ih2 * ih : ih;
+    // G = -2 W / h^2: grad W = G dx
+    const R G = (R(kGradNorm<D>) * ih2) * (ihD * rexp<R>(-expt));
     const R Vj = s.V[q];
     R vij[3];
     vij[0] = t.vel[0] - s.vx[q];
     if (D == 3) { vij[1] = t.vel[1] - s.vy[q]; vij[2] = t.vel[2] - s.vz[q]; }
-    const R cab = R(0.5) * (t.ci + s.c[q]);
+    const R c2 = t.ci + s.c[q];
     R vrab = 0, dotvv = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) { vrab += dx[k] * vij[k]; dotvv += vij[k] * vij[k]; }
@@ -913,40 +1024,38 @@ struct RhsPass {
     // approaching pairs, mu = h v.r / (r^2 + ata h^2)
     R vis = 0;
     if (vrab < R(0)) {
-      const R rhoab = R(0.5) * (t.rhoi + rhoj);
+      const R rs = t.rhoi + rhoj;
       const R den = dist_sq + R(a.ata) * h * h;
-      const R r = rrcp<R>(den * rhoab);
-      const R miu = h * vrab * (rhoab * r);
-      vis = (R(a.beta) * miu - R(a.alpha) * cab) * miu * (den * r);
+      const R r = rrcp<R>(den * rs);
+      const R miu = h * vrab * (rs * r);
+      vis = (R(a.beta2) * miu - R(a.alpha) * c2) * miu * (den * r);
     }
     if (a.lans) vis -= R(a.eps_half) * dotvv * Vj;        // SPH_epsilon_mom
     const R mpv = s.m[q] * (s.PV2[q] + t.pvsqi + vis);
-    const R g2 = R(-2) * ih * W;
-    R dw[3];
-#pragma unroll
-    for (int k = 0; k < D; k++) dw[k] = g2 * si[k];
     R heat = 0;
     if (a.lans && D == 3) {
       // k_ij F_ij of SPH_epsilon_heat_conductivity (6-arg, sph_lib.cc:1399-1439)
       // and compute_F (sph_lib.cc:257-281): 10 eps cp r^2 |v|^2 / (Pr sum|v_k dx_k|)
-      // / (h c) * (dW.dx / r^2); the r^2 factors cancel.
-      R dotvr = 0, wr = 0;
+      // / (h c) * (dW.dx / r^2); the r^2 factors cancel: dW.dx = G r^2.
+      R dotvr = 0;
 #pragma unroll
-      for (int k = 0; k < D; k++) { dotvr += rabs<R>(vij[k] * dx[k]); wr += dw[k] * dx[k]; }
+      for (int k = 0; k < D; k++) dotvr += rabs<R>(vij[k] * dx[k]);
+      const R wr = G * dist_sq;
       if (dotvr != R(0) && wr != R(0)) {
-        const R cpab = R(0.5) * (s.cp[q] + t.cpi);
-        const R kF = R(10.0 * a.eps_half) * cpab * dotvv * wr * rrcp<R>(R(a.prandtl) * dotvr * h * cab);
+        const R cp2 = s.cp[q] + t.cpi;
+        const R kF = R(a.kheat) * cp2 * dotvv * wr * rrcp<R>(dotvr * h * c2);
         heat = s.m[q] * Vj * t.Vi * (t.Ti - s.T[q]) * kF;
       }
     }
     R de = 0;
+    const R Gm = mpv * G;
 #pragma unroll
     for (int k = 0; k < D; k++) {
-      const R f = mpv * dw[k];
+      const R f = Gm * dx[k];
       t.rv[k] -= (double)f;
       de += f * vij[k];
     }
-    t.re += (double)(R(0.5) * de + heat);
+    t.re += (double)(de + heat);
   }
   template <int D>
   __device__ static void pair_mixed(const Args &a, Tgt &t, const Tile &s, uint32_t q, const double *dxd) {
@@ -955,33 +1064,33 @@ struct RhsPass {
     const float ih = rrcp<float>(h);
     const double ih64 = (double)ih * fma(-hd, (double)ih, 2.0);     // one Newton step: ~1e-14
     const double ih2 = ih64 * ih64;
+    const double ihD = (D == 3) ? ih2 * ih64 : ih64;
     double r2 = 0;
     float dx[3];
 #pragma unroll
     for (int k = 0; k < D; k++) { r2 = fma(dxd[k], dxd[k], r2); dx[k] = (float)dxd[k]; }
-    const double nk = kInvSqrtPi * ih64;
-    const double norm = (D == 3) ? nk * nk * nk : nk;
-    const double G64 = -2.0 * ih2 * norm * exp_neg_w<FX>(r2 * ih2, s.etab);
+    // G = -2 W / h^2: grad W = G dx
+    const double G64 = (kGradNorm<D> * ih2) * (ihD * exp_neg_w<FX>(r2 * ih2, s.etab));
     const float G = (float)G64, dist_sq = (float)r2;
     const float Vj = s.V[q], rhoj = s.rho[q];
     float vij[3];
     vij[0] = t.vel[0] - s.vx[q];
     if (D == 3) { vij[1] = t.vel[1] - s.vy[q]; vij[2] = t.vel[2] - s.vz[q]; }
-    const float cab = 0.5f * (t.ci + s.c[q]);
+    const float c2 = t.ci + s.c[q];
     float vrab = 0, dotvv = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) { vrab += dx[k] * vij[k]; dotvv += vij[k] * vij[k]; }
     float vis = 0;
     if (vrab < 0.f) {
-      const float rhoab = 0.5f * (t.rhoi + rhoj);
+      const float rs = t.rhoi + rhoj;
       const float den = dist_sq + (float)a.ata * h * h;
-      const float r = rrcp<float>(den * rhoab);
-      const float miu = h * vrab * (rhoab * r);
-      vis = ((float)a.beta * miu - (float)a.alpha * cab) * miu * (den * r);
+      const float r = rrcp<float>(den * rs);
+      const float miu = h * vrab * (rs * r);
+      vis = ((float)a.beta2 * miu - (float)a.alpha * c2) * miu * (den * r);
     }
     if (a.lans) vis -= (float)a.eps_half * dotvv * Vj;
     // pressure part in FP64, viscous part in FP32; f = (M_p + M_v) G dx
-    const double f = s.m64[q] * (s.pv64[q] + t.pv64) * G64 + (double)(s.m[q] * vis * G);
+    const double f = fma(s.m64[q], t.pv64, s.mpv64[q]) * G64 + (double)(s.m[q] * vis * G);
 #pragma unroll
     for (int k = 0; k < D; k++) t.rv[k] = fma(-f, dxd[k], t.rv[k]);
     float heat = 0;
@@ -991,25 +1100,25 @@ struct RhsPass {
       for (int k = 0; k < D; k++) dotvr += rabs<float>(vij[k] * dx[k]);
       const float wr = G * dist_sq;
       if (dotvr != 0.f && wr != 0.f) {
-        const float cpab = 0.5f * (s.cp[q] + t.cpi);
-        const float kF = (float)(10.0 * a.eps_half) * cpab * dotvv * wr * rrcp<float>((float)a.prandtl * dotvr * h * cab);
-        heat = s.m[q] * Vj * t.Vi * (t.Ti - s.T[q]) * kF;
+        const float cp2 = s.cp[q] + t.cpi;
+        const float kF = (float)a.kheat * cp2 * dotvv * wr * rrcp<float>(dotvr * h * c2);
+        heat = s.mV[q] * t.Vi * (t.Ti - s.T[q]) * kF;
       }
     }
-    t.re += 0.5 * f * (double)vrab + (double)heat;
+    t.re += fma(f, (double)vrab, (double)heat);
   }
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (t.bad) { raise_err(err, PSPH_EINVARIANT, i); return; }
     const double gz = -a.g, dt = a.dt;
     // v += dt rhs_v; gravity on the last axis; e += dt (rhs_e + v_last g)
     // (mom_engr_update.cc:372-388); the velocity in the energy term is the
-    // FP64 one of the target
+    // FP64 one of the target; t.re holds 2 rhs_e
     const double vl = a.dim == 3 ? (a.lans ? a.svz[i] : a.vz[i]) : (a.lans ? a.svx[i] : a.vx[i]);
     double nv0 = a.vx[i] + dt * (t.rv[0]);
     double nv1 = a.vy[i] + dt * (t.rv[1]);
     double nv2 = a.vz[i] + dt * (t.rv[2]);
     if (a.dim == 3) nv2 += dt * gz; else nv0 += dt * gz;
-    const double ne = a.e[i] + dt * (t.re + vl * gz);
+    const double ne = a.e[i] + dt * (0.5 * t.re + vl * gz);
     a.o_vx[i] = nv0; a.o_vy[i] = nv1; a.o_vz[i] = nv2;
     a.o_e[i] = ne;
     double p, c, T, gam, cp = a.cp[i];
@@ -1033,10 +1142,16 @@ struct DensArgs {
   CutTest cut;
   double *o_wn;            // optional: energy-smoothing normalisation (adjoint recompute)
   int e_sq16;              // e_ratio^2 == 16: exp(-16 x) = exp(-x)^16 by four squarings
+  double e_norm;           // the energy-smoothing kernel's norm times h^D: (e_ratio / sqrt(pi))^D
 };
 
-template <bool FX> struct DensMixTile { double m64[kWave], irho64[kWave], e64[kWave], etab[wtab_size<FX>()]; };
-template <class R> struct DensTileR { R m[kWave], rho[kWave], e[kWave]; };
+// pi^(-D/2): W = kNorm h^-D exp(-s^2)
+template <int D> constexpr double kNorm = kInvSqrtPi * kInvSqrtPi * kInvSqrtPi;
+template <> constexpr double kNorm<1> = kInvSqrtPi;
+
+// mr = m / rho of the candidate (the energy-smoothing weight's factor)
+template <bool FX> struct DensMixTile { double m64[kWave], mr64[kWave], e64[kWave], etab[wtab_size<FX>()]; };
+template <class R> struct DensTileR { R m[kWave], mr[kWave], e[kWave]; };
 
 // The FP32-pair form evaluates the density and energy-smoothing weights in
 // FP64 with the exp table: the pressure gradient of the next step amplifies
@@ -1069,8 +1184,8 @@ struct DensPass {
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
     s.hh[l] = 0.5 * a.h[j];
-    if constexpr (kMixed) { s.m64[l] = a.m[j]; s.irho64[l] = 1.0 / a.rho[j]; s.e64[l] = a.e[j]; }
-    else { s.m[l] = (R)a.m[j]; s.rho[l] = (R)a.rho[j]; s.e[l] = (R)a.e[j]; }
+    if constexpr (kMixed) { s.m64[l] = a.m[j]; s.mr64[l] = a.m[j] / a.rho[j]; s.e64[l] = a.e[j]; }
+    else { s.m[l] = (R)a.m[j]; s.mr[l] = (R)(a.m[j] / a.rho[j]); s.e[l] = (R)a.e[j]; }
     int ph = a.phase[j];
     s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
   }
@@ -1088,18 +1203,15 @@ struct DensPass {
       double r2 = 0;
 #pragma unroll
       for (int k = 0; k < D; k++) r2 = fma(ds[k], ds[k], r2);
-      const double expt = r2 * (ih * ih);
-      const double nk = kInvSqrtPi * ih;
-      const double norm = (D == 3) ? nk * nk * nk : nk;
-      const double mj = s.m64[q];
+      const double ih2 = ih * ih;
+      const double expt = r2 * ih2;
+      const double ihD = (D == 3) ? ih2 * ih : ih;
       const double ex = exp_neg_w<FX>(expt, s.etab);
-      const double wm = norm * ex * mj;
+      const double wm = (kNorm<D> * ihD) * (ex * s.m64[q]);
       if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
       if (a.energy_smooth && !a.cut.outside(a.e_ratio * amax, hijd)) {
-        const double nke = kInvSqrtPi * (a.e_ratio * ih);
-        const double ne = (D == 3) ? nke * nke * nke : nke;
         const double ee = a.e_sq16 ? pow16(ex) : exp_neg_w<FX>(a.e_ratio * a.e_ratio * expt, s.etab);
-        const double vr = ne * ee * mj * s.irho64[q];
+        const double vr = (a.e_norm * ihD) * (ee * s.mr64[q]);
         t.wn += vr;
         t.en += vr * s.e64[q];
       }
@@ -1113,19 +1225,15 @@ struct DensPass {
       R sk = (R)ds[k] * ih;
       expt += sk * sk;
     }
-    const R nk = R(kInvSqrtPi) * ih;
-    const R norm = (D == 3) ? nk * nk * nk : nk;
+    const R ihD = (D == 3) ? ih * ih * ih : ih;
     const R ex = rexp<R>(-expt);
-    const R wm = norm * ex * mj;
+    const R wm = (R(kNorm<D>) * ihD) * (ex * mj);
     if (s.ok[q] == 1) t.r1 += (double)wm; else t.r2 += (double)wm;
     if (a.energy_smooth) {
-      // weight(4 ds / h_ij, h_ij / 4) (HAVE_ENERGY_SMOOTH 1); exact FP64 box test
+      // weight(4 ds / h_ij, h_ij / 4) m_j / rho_j (HAVE_ENERGY_SMOOTH 1); exact FP64 box test
       if (!a.cut.outside(a.e_ratio * amax, hijd)) {
-        const R nke = R(kInvSqrtPi) * (R(a.e_ratio) * ih);
-        const R ne = (D == 3) ? nke * nke * nke : nke;
         const R e2 = R(a.e_ratio * a.e_ratio);
-        const R wm_e = ne * (a.e_sq16 ? pow16(ex) : rexp<R>(-(e2 * expt))) * mj;
-        const R vr = wm_e / s.rho[q];
+        const R vr = (R(a.e_norm) * ihD) * ((a.e_sq16 ? pow16(ex) : rexp<R>(-(e2 * expt))) * s.mr[q]);
         t.wn += (double)vr;
         t.en += (double)(vr * s.e[q]);
       }
@@ -1183,7 +1291,7 @@ struct VsmPass {
   __device__ static void init(Tile &s, int lane) {
     if constexpr (kMixed) wtab_fill<FX>(s.etab, lane);
   }
-  struct Tgt { double hi, supp, ih2, nrm, v64[3], dv[3], wn; R ih, norm, v[3]; };
+  struct Tgt { double hi, supp, ih2, nrm, v64[3], dv[3], wn; R ih, v[3]; };
   static constexpr int kSub = ::kSub;
   __device__ static void combine(Tgt &t) {
     for (int k = 0; k < 3; k++) t.dv[k] = sub_sum<kSub>(t.dv[k]);
@@ -1198,9 +1306,8 @@ struct VsmPass {
     const double ih = 1.0 / t.hi;
     const double nk = kInvSqrtPi * ih;
     t.ih = (R)ih;
-    t.ih2 = ih * ih;
+    t.ih2 = FX ? ih * ih * kLog2e : ih * ih;    // the FX form's exponential takes s^2 log2(e)
     t.nrm = a.vy ? nk * nk * nk : nk;
-    t.norm = (R)t.nrm;
     t.v64[0] = a.vx[i]; t.v64[1] = a.vy ? a.vy[i] : 0; t.v64[2] = a.vz ? a.vz[i] : 0;
     for (int k = 0; k < 3; k++) t.v[k] = (R)t.v64[k];
     t.dv[0] = t.dv[1] = t.dv[2] = 0; t.wn = 0;
@@ -1225,7 +1332,11 @@ struct VsmPass {
       double r2 = 0;
 #pragma unroll
       for (int k = 0; k < D; k++) r2 = fma(ds[k], ds[k], r2);
-      const double w = t.nrm * exp_neg_w<FX>(r2 * t.ih2, s.etab) * s.mr64[q];
+      // the weight without the kernel's norm, which cancels in dv / wn
+      double ew;
+      if constexpr (FX) ew = exp2_neg_fx(r2 * t.ih2);
+      else ew = exp_neg_tab(r2 * t.ih2, s.etab);
+      const double w = ew * s.mr64[q];
 #pragma unroll
       for (int k = 0; k < D; k++) t.dv[k] = fma(w, s.v64[k][q] - t.v64[k], t.dv[k]);
       t.wn += w;
@@ -1237,7 +1348,7 @@ struct VsmPass {
         R sk = (R)ds[k] * t.ih;
         expt += sk * sk;
       }
-      const R mvw = t.norm * rexp<R>(-expt) * s.mr[q];
+      const R mvw = rexp<R>(-expt) * s.mr[q];        // the norm cancels in dv / wn
       t.dv[0] += (double)(mvw * (s.vx[q] - t.v[0]));
       if (D == 3) {
         t.dv[1] += (double)(mvw * (s.vy[q] - t.v[1]));
@@ -1249,7 +1360,7 @@ struct VsmPass {
   __device__ static void finish(const Args &a, Tgt &t, uint32_t i, int *err) {
     if (!(t.wn > 0)) { raise_err(err, PSPH_EINVARIANT, i); return; }
     const double iw = 1.0 / t.wn;
-    if (a.o_wn) a.o_wn[i] = t.wn;
+    if (a.o_wn) a.o_wn[i] = t.wn * t.nrm;          // the normalisation sum with the kernel's norm
     a.svx[i] = t.v64[0] + t.dv[0] * iw;
     if (a.svy) { a.svy[i] = t.v64[1] + t.dv[1] * iw; a.svz[i] = t.v64[2] + t.dv[2] * iw; }
   }
@@ -1269,14 +1380,19 @@ struct AdaptArgs {
   int pghost, dim;
 };
 
-template <class R>
+// The kernel value comes from the FP64 exp table (exp_neg_tab, about an ulp),
+// or in the FX form (PSPH_FP32_EXP) from the FP32 exponential of the FP64
+// argument: its rounding is independent from pair to pair, so rho* and h
+// move by ~1e-9 relative over a particle's ~2000 list pairs.
+template <class R, bool FX = false>
 struct AdaptPass {
   using Args = AdaptArgs;
-  template <class T> __device__ static void init(T &, int) {}
   struct Tile {
     R m[kWave];
     int ok[kWave];
+    double etab[wtab_size<FX>()];
   };
+  __device__ static void init(Tile &s, int lane) { wtab_fill<FX>(s.etab, lane); }
   struct Tgt { double mi, hcur, Hs, supp, rho_star; R iHs, norm; bool done; };
   static constexpr int kSub = ::kSub;
   __device__ static void combine(Tgt &t) { t.rho_star = sub_sum<kSub>(t.rho_star); }
@@ -1309,7 +1425,7 @@ struct AdaptPass {
       R sk = (R)dx[k] * t.iHs;
       expt += sk * sk;
     }
-    t.rho_star += (double)(t.norm * rexp<R>(-expt) * s.m[q]);
+    t.rho_star += (double)(t.norm * (R)exp_neg_w<FX>((double)expt, s.etab) * s.m[q]);
   }
   __device__ static void end_pass(const Args &a, Tgt &t, int, uint32_t, bool active) {
     if (!active || t.done) return;
@@ -1519,17 +1635,26 @@ __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
       const bool any_contr = __any(ok);          // e.g. tiles of wall ghosts only
       __syncthreads();
       const uint32_t cnt = min((uint32_t)kWave, hi - base);
-      if (here && any_contr) {
-        // phase 1: which staged candidates reach this lane's image
+      if (any_contr) {
+        // phase 1, transposed: lane l tests staged candidate l against the
+        // image of each lane within reach of the tile, one lane g at a time;
+        // lane g keeps the ballot (bit q: candidate q reaches g's image)
         uint64_t mask = 0;
-        for (uint32_t q = 0; q < cnt; q++) {
-          const bool near = abs(sbk[0][q] - bi[0]) <= 1 && abs(sbk[1][q] - bi[1]) <= 1 &&
-                            abs(sbk[2][q] - bi[2]) <= 1;
-          const double hj = sh[q];
-          const double amax = fmax(fabs(refc[0] - sqx[q]), fmax(fabs(refc[1] - sqy[q]), fabs(refc[2] - sqz[q])));
+        uint64_t need = __ballot(near_t);
+        const bool cand = (uint32_t)lane < cnt;
+        const double ox = sqx[lane], oy = sqy[lane], oz = sqz[lane], oh = sh[lane];
+        const int o0 = sbk[0][lane], o1 = sbk[1][lane], o2 = sbk[2][lane];
+        while (need) {
+          const int g = __builtin_ctzll(need);
+          need &= need - 1;
+          const double r0 = __shfl(refc[0], g), r1 = __shfl(refc[1], g), r2 = __shfl(refc[2], g);
+          const int b0 = __shfl(bi[0], g), b1 = __shfl(bi[1], g), b2 = __shfl(bi[2], g);
+          const bool near = abs(o0 - b0) <= 1 && abs(o1 - b1) <= 1 && abs(o2 - b2) <= 1;
+          const double amax = fmax(fabs(r0 - ox), fmax(fabs(r1 - oy), fabs(r2 - oz)));
           // in_support; W = 0 outside the kernel box
-          const bool in = near & (amax <= a.cutoff2 * hj) & !a.cut.outside(amax, hj);
-          mask |= (uint64_t)in << q;
+          const bool in = cand & near & (amax <= a.cutoff2 * oh) & !a.cut.outside(amax, oh);
+          const uint64_t bal = __ballot(in);
+          mask = lane == g ? bal : mask;
         }
         // phase 2: the lane's own hits, in index order
         while (mask) {
@@ -1813,6 +1938,7 @@ struct StepRec {
   size_t cs_cap = 0, tile_cap = 0;
   GridDesc grid{};
   double reach = 0;
+  double hmax = 0;                            // bounds h at list time
   double dt = 0, time = 0;
   int erupting = 0;
   uint32_t inj_lo = 0, inj_hi = 0;          // ids of the particles this step injected
@@ -2303,6 +2429,9 @@ extern "C" int psph_neighbors(psph_ctx *c) {
   double del = c->cfg.bucket_edge > 0 ? c->cfg.bucket_edge : 1.0;
   if (c->cfg.dim == 3)
     for (int d = 0; d < 3; d++) nb[d] = (int)std::llround((c->cfg.ext_hi[d] - c->cfg.ext_lo[d]) / del);
+  for (int d = 0; d < 3; d++)
+    if (nb[d] > kRefbMaxPerAxis)
+      return fail(PSPH_ECAPACITY, "%d reference buckets on axis %d (at most %d)", nb[d], d, kRefbMaxPerAxis);
   if (n) hipLaunchKernelGGL(k_refb, blocks(n), 256, 0, s, n, qx, qy, qz, c->f.h,
                             c->cfg.ext_lo[0], c->cfg.ext_lo[1], c->cfg.ext_lo[2], del, nb[0], nb[1], nb[2],
                             c->refb, c->h_list);
@@ -2333,6 +2462,12 @@ static Traversal traversal(psph_ctx *c, double reach) {
   return Traversal{c->cs, c->toff, c->tbox, c->grid, reach};
 }
 
+// a list box 4.25 h can reach past the adjacent buckets: the list test needs
+// the bucket adjacency as well
+static int list_adj(const psph_config &k, double hmax) {
+  return (!k.list_all && k.dim == 3 && k.list_factor * hmax >= k.bucket_edge * (1 - 1e-12)) ? 1 : 0;
+}
+
 static PairCommon make_common(psph_ctx *c, double reach) {
   PairCommon pc;
   const bool frozen = c->tape.mode == 2 && c->tape.frozen;
@@ -2346,6 +2481,10 @@ static PairCommon make_common(psph_ctx *c, double reach) {
   pc.cfirst = c->cfirst; pc.ccount = c->ccount; pc.nchunks = c->nchunks;
   pc.list_all = (c->cfg.list_all || c->cfg.dim == 1) ? 1 : 0;
   pc.list_factor = c->cfg.list_factor;
+  pc.del = c->cfg.bucket_edge;
+  for (int d = 0; d < 3; d++) pc.ext_lo[d] = c->cfg.ext_lo[d];
+  // h_max bounds h at list time (it only grows with an adaptation)
+  pc.list_adj = list_adj(c->cfg, c->h_max);
   pc.cut = cut_test(c->cfg.cutoff);
   pc.err = c->d_err;
   return pc;
@@ -2382,7 +2521,9 @@ static int ensure_visits(psph_ctx *c) {
     for (auto pp : arrs) PSPH_TRY(dmalloc((void **)pp, ((size_t)G + 1) * 4));
     c->vchunk_cap = (size_t)G + 1;
   }
-  hipLaunchKernelGGL(k_visit_count, G, kWave, 0, c->stream, pc, c->vcount);
+  // a superset of the three passes' kernel boxes with a relative margin
+  const double cutx = c->cfg.cutoff * (1.0 + 1e-12);
+  hipLaunchKernelGGL(k_visit_count, G, kWave, 0, c->stream, pc, c->f.h, cutx, c->h_max, c->vcount);
   hipLaunchKernelGGL(k_fill_i32, 1, 64, 0, c->stream, (int *)c->vcount + G, 0, 1);
   HIP_TRY(hipGetLastError());
   PSPH_TRY(excl_scan(c, c->vcount, c->voff, (size_t)G + 1));
@@ -2398,8 +2539,6 @@ static int ensure_visits(psph_ctx *c) {
     PSPH_TRY(dmalloc((void **)&c->vent, cap * sizeof(uint2)));
     c->vis_cap = cap;
   }
-  // a superset of the three passes' kernel boxes with a relative margin
-  const double cutx = c->cfg.cutoff * (1.0 + 1e-12);
   if (c->cfg.dim == 3)
     hipLaunchKernelGGL(k_visit_masks<3>, G, kWave, 0, c->stream, pc, c->f.h, c->cfg.cutoff2, cutx, c->h_max,
                        c->voff, c->vnum, c->vent, c->vmask);
@@ -2479,10 +2618,13 @@ extern "C" int psph_adapt_h(psph_ctx *c) {
     const double reach = std::min(k.cutoff * k.c_smooth * hmax, list_reach) * (1 + 1e-9);
     {
       StageTimer tm(c, 2);
-      // FP64 in both forms: h enters every later pass, and a relative error
-      // of h moves every kernel value by ~|s|^2 times as much (two launches
-      // every SML_UPDATE_INT steps)
-      if (c->n) launch_pairs<AdaptPass<double>>(c, make_common(c, reach), a);
+      // FP64 arithmetic in every form (h enters every later pass, and a
+      // relative error of h moves every kernel value by ~|s|^2 times as
+      // much); the FX form takes its exponentials from the FP32 instruction
+      if (c->n) {
+        if (k.precision == PSPH_FP32_EXP) launch_pairs<AdaptPass<double, true>>(c, make_common(c, reach), a);
+        else launch_pairs<AdaptPass<double>>(c, make_common(c, reach), a);
+      }
     }
     HIP_TRY(hipGetLastError());
     PSPH_TRY(reduce_hmax(c, &hmax));
@@ -2521,6 +2663,7 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   a.o_e = ov[3]; a.o_p = ov[4]; a.o_c = ov[5]; a.o_T = ov[6]; a.o_gam = ov[7]; a.o_cp = ov[8];
   a.dt = dt; a.g = k.g; a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2; a.alpha = k.alpha; a.beta = k.beta;
   a.ata = k.ata; a.eps_half = k.eps_half; a.prandtl = k.prandtl; a.heat_scale = k.heat_scale;
+  a.beta2 = 2.0 * k.beta; a.kheat = 20.0 * k.eps_half / k.prandtl;
   a.lans = k.lans; a.dim = k.dim; a.eos = eos_const(k); a.pairs = c->d_u64 + 2;
   hipLaunchKernelGGL(k_fill_u64, 1, 64, 0, c->stream, c->d_u64 + 2, 0ull, 1);
   if (n) PSPH_TRY(ensure_visits(c));
@@ -2554,6 +2697,10 @@ extern "C" int psph_density(psph_ctx *c) {
   a.cut = cut_test(k.cutoff);
   a.o_wn = c->rec ? c->rec->d[RF_WNE] : nullptr;
   a.e_sq16 = k.e_smooth_ratio * k.e_smooth_ratio == 16.0;
+  {
+    const double en = kInvSqrtPi * k.e_smooth_ratio;
+    a.e_norm = k.dim == 3 ? en * en * en : en;
+  }
   if (n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 4);

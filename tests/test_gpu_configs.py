@@ -202,13 +202,48 @@ def test_weak_plume_4M_500_steps_against_chaos_floor():
 
 
 def test_adjoint_512k_200_steps_checkpoint_20():
+    """The adjoint configuration: 512k real particles, 200 steps, a checkpoint
+    every 20; backward at most 3x the plain forward.
+
+    Central differences through the frozen replay: over the first 20 steps
+    the 16 seeded directions at eps 1e-6 agree to 1e-4 (measured 1.9e-6).
+    Over the full 200 steps the map is chaotic (dH/d ln v_vent ~ 3e6 m): the
+    two-point quotient leaves the linear range at eps 1e-6 (3e-2 on the same
+    directions) and FP64 rounding of H, amplified like any perturbation, takes
+    over below eps ~1e-9 (0.11 at 1e-10, 0.35 at 1e-11).  At the bottom of
+    that curve a four-point stencil at eps 3e-9 agrees to 2e-3 (measured
+    8.4e-4), and the vent velocity alone, with the largest signal, at eps
+    1e-10 to 1e-4."""
     from psph.torch_adjoint import build_state
     cfg = weak_plume_config(5.12e5)
     met = psph.meteo_standard()
     theta0 = (cfg.vent_v, cfg.vent_T, 1.0 - cfg.ng0)
+    theta = np.array(theta0)
     _, P = build_state(cfg, met, SEED, 0.01, *theta0, None)
+    rng = np.random.default_rng(SEED + 1000)
+    dirs = []
+    for d in range(16):
+        v = rng.standard_normal(3 + 3 * len(P))
+        dirs.append(v / np.linalg.norm(v))
     N = 200
     with psph.Solver(cfg, len(P) + 400 * N + 50000) as S:
+
+        def H(n, eps, v):
+            dth, dv = v[:3], v[3:].reshape(len(P), 3)
+            c, Pp = build_state(cfg, met, SEED, 0.01, *(theta * (1 + eps * dth)), eps * theta[0] * dv)
+            S.set_vent(c.vent_v, c.vent_T, c.ng0)
+            S.load(Pp)
+            S.replay(n)
+            return S.diag_height()
+
+        def ad_dir(r, v):
+            g = np.array([r["dH_dvent_v"], r["dH_dvent_T"], r["dH_dmass_frac"]])
+            return float(np.sum(v[:3] * theta * g) + theta[0] * np.sum(v[3:].reshape(len(P), 3) * r["grad_v"]))
+
+        def reset():
+            S.set_vent(*theta0[:2], 1.0 - theta0[2])
+            S.load(P)
+
         S.load(P)
         S.step()                      # first-touch warm-up
         S.load(P)
@@ -219,41 +254,40 @@ def test_adjoint_512k_200_steps_checkpoint_20():
         fwd_ms = 1e3 * (time.perf_counter() - t0)
         H_plain = S.diag_height()
         say(f"adjoint 512k: {len(P)} particles; plain forward {N} steps {fwd_ms:.0f} ms; adjoint K=20")
-        S.load(P)
+        # 20 steps: the 16 directions at eps 1e-6
+        reset()
+        r20 = S.adjoint_run(20, 20, grads=True)
+        worst20 = 0.0
+        for d, v in enumerate(dirs):
+            fdv = (H(20, 1e-6, v) - H(20, -1e-6, v)) / 2e-6
+            worst20 = max(worst20, abs(ad_dir(r20, v) - fdv) / abs(fdv))
+        say(f"20 steps, 16 directions at eps 1e-6: worst relative difference {worst20:.2e}")
+        assert worst20 <= 1e-4, worst20
+        # 200 steps
+        reset()
         r = S.adjoint_run(N, 20, grads=True)
         say(f"K=20: H={r['H']:.9f} dH/dv={r['dH_dvent_v']:.6e} dH/dT={r['dH_dvent_T']:.6e} "
             f"dH/dmf={r['dH_dmass_frac']:.6e} backward {r['backward_ms']:.0f} ms")
         assert r["H"] == H_plain
-        S.load(P)
+        reset()
         S.replay(N)
         assert abs(S.diag_height() - r["H"]) <= 1e-9 * abs(r["H"])
-        # central differences through the frozen replay on the 16 seeded
-        # directions over (vent v, T, mass fraction, every initial velocity)
-        theta = np.array(theta0)
-        g = np.array([r["dH_dvent_v"], r["dH_dvent_T"], r["dH_dmass_frac"]])
-        rng = np.random.default_rng(SEED + 1000)
-        eps, worst = 1e-6, 0.0
-        for d in range(16):
-            v = rng.standard_normal(3 + 3 * len(P))
-            v /= np.linalg.norm(v)
-            dth, dv = v[:3], v[3:].reshape(len(P), 3)
-            hs = []
-            for sgn in (1, -1):
-                c, Pp = build_state(cfg, met, SEED, 0.01, *(theta * (1 + sgn * eps * dth)), sgn * eps * theta[0] * dv)
-                S.set_vent(c.vent_v, c.vent_T, c.ng0)
-                S.load(Pp)
-                S.replay(N)
-                hs.append(S.diag_height())
-            fdv = (hs[0] - hs[1]) / (2 * eps)
-            ad = float(np.sum(dth * theta * g) + theta[0] * np.sum(dv * r["grad_v"]))
-            rel = abs(ad - fdv) / abs(fdv)
+        e_v = np.zeros(3 + 3 * len(P))
+        e_v[0] = 1.0
+        fdv = (H(N, 1e-10, e_v) - H(N, -1e-10, e_v)) / 2e-10
+        rel_v = abs(ad_dir(r, e_v) - fdv) / abs(fdv)
+        say(f"200 steps, vent velocity at eps 1e-10: AD {ad_dir(r, e_v):.9e} FD {fdv:.9e} rel {rel_v:.2e}")
+        eps, worst = 3e-9, 0.0
+        for d, v in enumerate(dirs):
+            fd4 = (8 * (H(N, eps, v) - H(N, -eps, v)) - (H(N, 2 * eps, v) - H(N, -2 * eps, v))) / (12 * eps)
+            rel = abs(ad_dir(r, v) - fd4) / abs(fd4)
             worst = max(worst, rel)
-            say(f"direction {d}: AD {ad:.9e} FD {fdv:.9e} rel {rel:.2e}")
-        S.set_vent(*theta0[:2], 1.0 - theta0[2])
-        say(f"FD at eps 1e-6 over {N} steps: worst relative difference {worst:.2e}")
-        assert worst <= 1e-4, worst
+            say(f"direction {d}: AD {ad_dir(r, v):.9e} FD(4-point, eps 3e-9) {fd4:.9e} rel {rel:.2e}")
+        say(f"200 steps, 16 directions, four-point stencil at eps 3e-9: worst relative difference {worst:.2e}")
+        assert rel_v <= 1e-4, rel_v
+        assert worst <= 2e-3, worst
         say("adjoint K=200 (one segment, no recompute)")
-        S.load(P)
+        reset()
         r1 = S.adjoint_run(N, N, grads=True)
     for k in ("H", "dH_dvent_v", "dH_dvent_T", "dH_dmass_frac"):
         assert abs(r1[k] - r[k]) <= 1e-12 * abs(r[k]), (k, r1[k], r[k])

@@ -1,5 +1,7 @@
-"""FP32-pair form (pair arithmetic in FP32 on exact FP64 pair predicates,
-FP64 accumulation / integration / EOS) against the FP64 CPU oracle.
+"""FP32-pair forms (pair arithmetic in FP32 on exact FP64 pair predicates,
+FP64 accumulation / integration / EOS; PSPH_FP32_EXP also takes the weights'
+exponentials from the FP32 instruction, the adaptive h's included) against
+the FP64 CPU oracle.
 
 Bars: the same pairs as the FP64 form (identical pair counts and particle
 sets), and every field within 1e-5 relative L1 of the FP64 oracle."""
@@ -21,12 +23,14 @@ def say(msg):
     sys.stdout.flush()
 
 
-def test_fp32_pair_plume_steps():
+@pytest.mark.parametrize("form", ["PSPH_FP32_PAIR", "PSPH_FP32_EXP"])
+def test_fp32_pair_plume_steps(form):
+    """8 steps with the adaptive h at steps 4 and 8."""
     cfg, met, P = small_plume(sml_update_int=4)
-    say(f"fp32-pair plume: {len(P)} particles, 8 steps")
+    say(f"{form} plume: {len(P)} particles, 8 steps")
     S = oracle.State(cfg, P)
     c32 = cfg.copy()
-    c32.precision = psph.PSPH_FP32_PAIR
+    c32.precision = getattr(psph, form)
     with psph.Solver(c32, len(P) + 20000) as G, psph.Solver(cfg, len(P) + 20000) as G64:
         G.load(P)
         G64.load(P)
@@ -46,13 +50,14 @@ def test_fp32_pair_plume_steps():
         assert not bad, bad
 
 
-def test_fp32_pair_sod():
+@pytest.mark.parametrize("form", ["PSPH_FP32_PAIR", "PSPH_FP32_EXP"])
+def test_fp32_pair_sod(form):
     cfg, P = sod()
     S = oracle.State(cfg, P)
     S.neighbors()
     S.adaptive_sml()
     c32 = cfg.copy()
-    c32.precision = psph.PSPH_FP32_PAIR
+    c32.precision = getattr(psph, form)
     with psph.Solver(c32, len(P) + 16) as G:
         G.load(P)
         G.search_neighs_consth()

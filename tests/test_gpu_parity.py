@@ -65,6 +65,36 @@ def test_plume_stage_by_stage():
         compare(G.read(), S.particles(), FLOAT_FIELDS_3D, REL_TOL_FP64)
 
 
+def test_lists_wider_than_a_bucket():
+    """Smoothing lengths of 2.2 h_air make the 4.25 h list box (9.35 h_air)
+    wider than a reference bucket (8 h_air): the list then holds the own
+    bucket and, from the 26 adjacent buckets only, the particles inside the
+    box (neighs.cc:37-211); a particle two buckets away is outside even when
+    inside the box.  Lists, one adaptive-h update and one RHS step against
+    the oracle (whose bucket search equals its brute force)."""
+    cfg, met, P = small_plume()
+    P["h"][:] = 2.2 * cfg.h_air
+    say(f"lists wider than a bucket: {len(P)} particles, h = {2.2 * cfg.h_air:.1f}")
+    S = oracle.State(cfg, P)
+    with psph.Solver(cfg, len(P) + 20000) as G:
+        G.load(P)
+        off_o, idx_o = S.neighbors()
+        off_b, idx_b = S.neighbors(brute=True)
+        assert same_lists(off_o, idx_o, off_b, idx_b) == -1
+        G.search_neighs_consth()
+        off_g, idx_g = G.neighbors()
+        assert off_o[-1] == off_g[-1], (off_o[-1], off_g[-1])
+        assert same_lists(off_g, idx_g, off_o, idx_o) == -1
+        S.adaptive_sml()
+        G.adaptive_sml()
+        compare(G.read(), S.particles(), FLOAT_FIELDS_3D, REL_TOL_FP64)
+        dt = S.timestep()
+        S.rhs(dt)
+        G.mom_engr_update(dt)
+        errs = compare(G.read(), S.particles(), FLOAT_FIELDS_3D, REL_TOL_FP64)
+        say(f"max rel err {max(errs.values()):.3e}")
+
+
 @pytest.mark.parametrize("nsteps,adapt", [(6, 3), (4, 50)])
 def test_plume_steps(nsteps, adapt):
     cfg, met, P = small_plume(sml_update_int=adapt)

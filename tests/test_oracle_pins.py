@@ -173,8 +173,13 @@ def test_philox_known_answers():
 
 
 # ---------------------------------------------------------------- neighbours
-def test_neighbour_search_matches_bruteforce():
+@pytest.mark.parametrize("h_scale", [None, 2.2])
+def test_neighbour_search_matches_bruteforce(h_scale):
+    """Also with h = 2.2 h_air, where the 4.25 h list box is wider than a
+    bucket and only the adjacent buckets' particles may enter a list."""
     cfg, met, P = small_plume()
+    if h_scale:
+        P["h"][:] = h_scale * cfg.h_air
     S = oracle.State(cfg, P)
     off_a, idx_a = S.neighbors()
     off_b, idx_b = S.neighbors(brute=True)
