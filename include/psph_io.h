@@ -79,6 +79,45 @@ int psph_h5part_read(const char *path, int64_t group, const char *name, int32_t 
 int psph_csv_write_soa(const char *path, const psph_soa *p, size_t n);
 int psph_write_csv(psph_ctx *ctx, const char *path);
 
+/* ---- the preprocessor's grid file funky%04d.h5 --------------------------
+ * The grid preprocessor (preprocess.cc, createfunky.cc:32-71) writes per
+ * process "/hashtable_constants" (6 float64: x min, x max, y min, y max,
+ * z min, z max of the extended domain, preprocess.cc:136-141), "/Buckets"
+ * (one compound record per bucket, in partition order) and
+ * "/partition_table" (uint32, 4 per bucket: partition key, bucket key), and
+ * Read_Grid (dataread.cc:248-499) reads them back.  The record's compound
+ * type (GH5_bucketstruct, hdf5calls.cc:175-203) has the size and member
+ * offsets of BucketStruct (buckstr.h:14-26) but declares each array member
+ * as a scalar, so a file holds, per bucket, Key (key[0]), neighs (the first
+ * neighbour's key[0]), buckettype (1 underground, 2 mixed, 3 overground, 4
+ * pressure boundary), myproc, neigh_proc (the first neighbour's process),
+ * xcoord, ycoord, zcoord (the lower corner) and elev (the ground elevation,
+ * elev[0]).  This build derives its buckets from the deck (psph_config_domain
+ * and the preprocessor's rules); these functions read a grid file, check a
+ * configuration against it, and write one. */
+typedef struct psph_grid_info {
+  double ext_lo[3], ext_hi[3];   /* /hashtable_constants                         */
+  double bucket_edge;            /* spacing of the buckets' lower corners        */
+  int32_t nb[3];                 /* buckets per axis                             */
+  int64_t n_buckets;             /* records in /Buckets                          */
+  int64_t n_type[5];             /* records per buckettype 0..4                  */
+  int32_t nproc;                 /* 1 + the largest myproc                       */
+  int32_t flat;                  /* every record holds the same elev             */
+  double ground;                 /* that elevation (the first record's if not)   */
+  int64_t n_partition;           /* uint32 words in /partition_table             */
+} psph_grid_info;
+int psph_grid_read(const char *path, psph_grid_info *out);
+/* Checks cfg (after psph_config_domain) against a grid file: the same
+ * extended domain and bucket edge (1e-9 relative), every bucket of the file
+ * at its place with the type this build derives, and a flat ground at
+ * cfg->dom_lo[2].  PSPH_EINVAL on a mismatch; PSPH_ENOTSUP for a grid split
+ * over several processes or with varying ground elevations (terrain), which
+ * this build does not run. */
+int psph_grid_check(const char *path, const psph_config *cfg);
+/* Writes the one-process grid file of cfg's domain (the preprocessor with
+ * np = 1 and a flat ground at cfg->dom_lo[2]) in the layout above. */
+int psph_grid_write(const char *path, const psph_config *cfg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,7 +4,10 @@
 // particles, fail / psph_last_error for errors).
 #include <hdf5.h>
 
+#include <algorithm>
 #include <cinttypes>
+#include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -226,4 +229,287 @@ extern "C" int psph_write_csv(psph_ctx *ctx, const char *path) {
   Snapshot s;
   PSPH_TRY(s.fill(ctx));
   return psph_csv_write_soa(path, &s.soa, s.P.size());
+}
+
+// ---------------------------------------------------------------------------
+// Grid files (funky%04d.h5): createfunky (createfunky.cc:32-71) and Read_Grid
+// (dataread.cc:248-499).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kKeyLen = 2;    // bucket keys (the preprocessor's KEYLENGTH)
+constexpr int kNeigh = 27;    // NEIGH_SIZE (constant.h:117)
+
+// The preprocessor's record (buckstr.h:14-26): the file's compound type takes
+// its size and member offsets from it.
+struct BucketRecord {
+  uint32_t key[kKeyLen];
+  uint32_t neighs[kNeigh * kKeyLen];
+  int32_t buckettype;
+  int32_t myproc;
+  int32_t neigh_proc[kNeigh];
+  int32_t bucket_index[6];
+  double xcoord[2], ycoord[2], zcoord[2];
+  double elev[4];
+};
+static_assert(sizeof(BucketRecord) == 448 && offsetof(BucketRecord, buckettype) == 224 &&
+                  offsetof(BucketRecord, neigh_proc) == 232 && offsetof(BucketRecord, xcoord) == 368 &&
+                  offsetof(BucketRecord, elev) == 416,
+              "the record must keep BucketStruct's layout");
+
+// GH5_bucketstruct (hdf5calls.cc:175-203): one scalar per member, at the
+// record's offsets
+hid_t bucket_file_type() {
+  const hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(BucketRecord));
+  H5Tinsert(t, "Key", offsetof(BucketRecord, key), H5T_NATIVE_UINT);
+  H5Tinsert(t, "neighs", offsetof(BucketRecord, neighs), H5T_NATIVE_UINT);
+  H5Tinsert(t, "buckettype", offsetof(BucketRecord, buckettype), H5T_NATIVE_INT);
+  H5Tinsert(t, "myproc", offsetof(BucketRecord, myproc), H5T_NATIVE_INT);
+  H5Tinsert(t, "neigh_proc", offsetof(BucketRecord, neigh_proc), H5T_NATIVE_INT);
+  H5Tinsert(t, "xcoord", offsetof(BucketRecord, xcoord), H5T_NATIVE_DOUBLE);
+  H5Tinsert(t, "ycoord", offsetof(BucketRecord, ycoord), H5T_NATIVE_DOUBLE);
+  H5Tinsert(t, "zcoord", offsetof(BucketRecord, zcoord), H5T_NATIVE_DOUBLE);
+  H5Tinsert(t, "elev", offsetof(BucketRecord, elev), H5T_NATIVE_DOUBLE);
+  return t;
+}
+
+// what a record holds, read by member name
+struct BucketRead {
+  uint32_t key, neighs;
+  int32_t buckettype, myproc, neigh_proc;
+  double x, y, z, elev;
+};
+hid_t bucket_read_type() {
+  const hid_t t = H5Tcreate(H5T_COMPOUND, sizeof(BucketRead));
+  H5Tinsert(t, "Key", HOFFSET(BucketRead, key), H5T_NATIVE_UINT32);
+  H5Tinsert(t, "neighs", HOFFSET(BucketRead, neighs), H5T_NATIVE_UINT32);
+  H5Tinsert(t, "buckettype", HOFFSET(BucketRead, buckettype), H5T_NATIVE_INT32);
+  H5Tinsert(t, "myproc", HOFFSET(BucketRead, myproc), H5T_NATIVE_INT32);
+  H5Tinsert(t, "neigh_proc", HOFFSET(BucketRead, neigh_proc), H5T_NATIVE_INT32);
+  H5Tinsert(t, "xcoord", HOFFSET(BucketRead, x), H5T_NATIVE_DOUBLE);
+  H5Tinsert(t, "ycoord", HOFFSET(BucketRead, y), H5T_NATIVE_DOUBLE);
+  H5Tinsert(t, "zcoord", HOFFSET(BucketRead, z), H5T_NATIVE_DOUBLE);
+  H5Tinsert(t, "elev", HOFFSET(BucketRead, elev), H5T_NATIVE_DOUBLE);
+  return t;
+}
+
+// determine_face_type (preprocess.cc:386-399)
+int face_of(double crd, double hi, double lo) { return crd < lo ? -1 : (crd > hi ? 1 : 0); }
+
+int dataset_len(hid_t fp, const char *name, hsize_t *n) {
+  if (H5Lexists(fp, name, H5P_DEFAULT) <= 0) return fail(PSPH_EIO, "grid file: no dataset %s", name);
+  H5Id ds(H5Dopen2(fp, name, H5P_DEFAULT), H5Dclose);
+  if (ds < 0) return fail(PSPH_EIO, "grid file: cannot open %s", name);
+  H5Id space(H5Dget_space(ds), H5Sclose);
+  if (H5Sget_simple_extent_ndims(space) != 1) return fail(PSPH_EIO, "grid file: %s is not one-dimensional", name);
+  H5Sget_simple_extent_dims(space, n, nullptr);
+  return PSPH_OK;
+}
+
+int grid_load(const char *path, psph_grid_info *info, std::vector<BucketRead> &rec) {
+  if (!path || !info) return fail(PSPH_EINVAL, "grid file: null argument");
+  quiet();
+  H5Id fp(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+  if (fp < 0) return fail(PSPH_EIO, "grid file: cannot open %s", path);
+  std::memset(info, 0, sizeof *info);
+  hsize_t nh = 0, nb = 0, np = 0;
+  PSPH_TRY(dataset_len(fp, "/hashtable_constants", &nh));
+  if (nh != 6) return fail(PSPH_EIO, "grid file: /hashtable_constants holds %llu values, not 6", (unsigned long long)nh);
+  double ht[6];
+  {
+    H5Id ds(H5Dopen2(fp, "/hashtable_constants", H5P_DEFAULT), H5Dclose);
+    if (H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, ht) < 0)
+      return fail(PSPH_EIO, "grid file: cannot read /hashtable_constants");
+  }
+  for (int d = 0; d < 3; d++) { info->ext_lo[d] = ht[2 * d]; info->ext_hi[d] = ht[2 * d + 1]; }
+  PSPH_TRY(dataset_len(fp, "/Buckets", &nb));
+  rec.resize(nb);
+  if (nb) {
+    H5Id ds(H5Dopen2(fp, "/Buckets", H5P_DEFAULT), H5Dclose);
+    H5Id t(bucket_read_type(), H5Tclose);
+    if (H5Dread(ds, t, H5S_ALL, H5S_ALL, H5P_DEFAULT, rec.data()) < 0)
+      return fail(PSPH_EIO, "grid file: cannot read /Buckets");
+  }
+  PSPH_TRY(dataset_len(fp, "/partition_table", &np));
+  info->n_buckets = (int64_t)nb;
+  info->n_partition = (int64_t)np;
+  info->flat = 1;
+  info->ground = nb ? rec[0].elev : 0.0;
+  int32_t maxproc = -1;
+  double edge = 0;
+  for (const BucketRead &r : rec) {
+    if (r.buckettype >= 0 && r.buckettype <= 4) info->n_type[r.buckettype]++;
+    maxproc = std::max(maxproc, r.myproc);
+    if (r.elev != info->ground) info->flat = 0;
+    const double dx = r.x - info->ext_lo[0];
+    if (dx > 0 && (edge == 0 || dx < edge)) edge = dx;
+  }
+  info->nproc = maxproc + 1;
+  // the lower corners lie on ext_lo + i del: the smallest positive offset
+  // from the origin on x is del when the file holds a bucket at i = 1
+  info->bucket_edge = edge;
+  for (int d = 0; d < 3; d++)
+    info->nb[d] = edge > 0 ? (int32_t)std::llround((info->ext_hi[d] - info->ext_lo[d]) / edge) : 0;
+  return PSPH_OK;
+}
+
+}  // namespace
+
+extern "C" int psph_grid_read(const char *path, psph_grid_info *out) {
+  std::vector<BucketRead> rec;
+  return grid_load(path, out, rec);
+}
+
+extern "C" int psph_grid_check(const char *path, const psph_config *cfg) {
+  if (!cfg) return fail(PSPH_EINVAL, "grid file: null argument");
+  if (cfg->dim != 3) return fail(PSPH_EINVAL, "grid file: the grid describes a 3D domain");
+  psph_grid_info g;
+  std::vector<BucketRead> rec;
+  PSPH_TRY(grid_load(path, &g, rec));
+  if (g.nproc > 1)
+    return fail(PSPH_ENOTSUP, "grid file %s: buckets of %d processes; one GPU runs the whole domain (preprocess "
+                "with one process)", path, g.nproc);
+  if (!g.flat) return fail(PSPH_ENOTSUP, "grid file %s: the ground elevation varies (terrain), not supported", path);
+  const double del = cfg->bucket_edge;
+  for (int d = 0; d < 3; d++) {
+    const double tol = 1e-9 * (cfg->ext_hi[d] - cfg->ext_lo[d]);
+    if (std::fabs(g.ext_lo[d] - cfg->ext_lo[d]) > tol || std::fabs(g.ext_hi[d] - cfg->ext_hi[d]) > tol)
+      return fail(PSPH_EINVAL, "grid file %s: extended domain [%.17g, %.17g] on axis %d, the config's [%.17g, %.17g]",
+                  path, g.ext_lo[d], g.ext_hi[d], d, cfg->ext_lo[d], cfg->ext_hi[d]);
+  }
+  if (std::fabs(g.bucket_edge - del) > 1e-9 * del)
+    return fail(PSPH_EINVAL, "grid file %s: bucket edge %.17g, the config's %.17g", path, g.bucket_edge, del);
+  if (std::fabs(g.ground - cfg->dom_lo[2]) > 1e-9 * del)
+    return fail(PSPH_EINVAL, "grid file %s: ground at %.17g, the config's dom_lo[2] %.17g", path, g.ground, cfg->dom_lo[2]);
+  psph::DomainBuckets B;
+  psph::domain_buckets(*cfg, B);
+  if ((int64_t)B.size() != g.n_buckets)
+    return fail(PSPH_EINVAL, "grid file %s: %lld buckets, the config's grid %zu", path, (long long)g.n_buckets, B.size());
+  std::vector<uint8_t> seen(B.size(), 0);
+  for (const BucketRead &r : rec) {
+    const double c[3] = {r.x, r.y, r.z};
+    int b[3];
+    for (int d = 0; d < 3; d++) {
+      const double f = (c[d] - cfg->ext_lo[d]) / del;
+      b[d] = (int)std::llround(f);
+      if (b[d] < 0 || b[d] >= B.nb[d] || std::fabs(f - b[d]) > 1e-6)
+        return fail(PSPH_EINVAL, "grid file %s: a bucket corner (%.17g, %.17g, %.17g) off the config's grid", path, c[0],
+                    c[1], c[2]);
+    }
+    const size_t q = B.idx(b[0], b[1], b[2]);
+    if (seen[q]++) return fail(PSPH_EINVAL, "grid file %s: bucket (%d, %d, %d) twice", path, b[0], b[1], b[2]);
+    if (r.buckettype != B.type[q])
+      return fail(PSPH_EINVAL, "grid file %s: bucket (%d, %d, %d) of type %d, this build derives %d", path, b[0], b[1],
+                  b[2], r.buckettype, (int)B.type[q]);
+  }
+  return PSPH_OK;
+}
+
+extern "C" int psph_grid_write(const char *path, const psph_config *cfg) {
+  if (!path || !cfg) return fail(PSPH_EINVAL, "grid file: null argument");
+  if (cfg->dim != 3) return fail(PSPH_EINVAL, "grid file: the grid describes a 3D domain");
+  quiet();
+  psph::DomainBuckets B;
+  psph::domain_buckets(*cfg, B);
+  const int nx = B.nb[0], ny = B.nb[1], nz = B.nb[2];
+  const double del = cfg->bucket_edge;
+  const double *lo = cfg->ext_lo, *hi = cfg->ext_hi;
+  std::vector<BucketRecord> bg(B.size());
+  std::memset(bg.data(), 0, bg.size() * sizeof(BucketRecord));
+  auto key_at = [&](int i, int j, int k, uint32_t key[kKeyLen]) {
+    // bucket centre normalised to the hash-table domain, HSFC3d
+    // (preprocess.cc:196-203; the two-word key of THSFC3d's words 0-1)
+    const double cr[3] = {((lo[0] + i * del) + (lo[0] + (i + 1) * del)) * 0.5,
+                          ((lo[1] + j * del) + (lo[1] + (j + 1) * del)) * 0.5,
+                          ((lo[2] + k * del) + (lo[2] + (k + 1) * del)) * 0.5};
+    double normc[3];
+    for (int l = 0; l < 3; l++) normc[l] = (cr[l] - lo[l]) / (hi[l] - lo[l]);
+    uint32_t k3[3];
+    psph::thsfc3d(normc, 0, k3);
+    key[0] = k3[0];
+    key[1] = k3[1];
+  };
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        BucketRecord &r = bg[B.idx(i, j, k)];
+        const int b[3] = {i, j, k};
+        for (int l = 0; l < 2; l++) {
+          r.xcoord[l] = lo[0] + (i + l) * del;
+          r.ycoord[l] = lo[1] + (j + l) * del;
+          r.zcoord[l] = lo[2] + (k + l) * del;
+        }
+        for (int l = 0; l < 4; l++) r.elev[l] = cfg->dom_lo[2];      // flat ground (preprocess.cc:160-162)
+        for (int d = 0; d < 3; d++) {
+          const double c0 = lo[d] + b[d] * del, c1 = lo[d] + (b[d] + 1) * del;
+          r.bucket_index[2 * d] = face_of(c0, cfg->dom_hi[d], cfg->dom_lo[d]);
+          r.bucket_index[2 * d + 1] = face_of(c1, cfg->dom_hi[d], cfg->dom_lo[d]);
+        }
+        r.buckettype = B.type[B.idx(i, j, k)];
+        r.myproc = 0;
+        key_at(i, j, k, r.key);
+      }
+  // neighbours (preprocess.cc:280-315)
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        BucketRecord &r = bg[B.idx(i, j, k)];
+        if (!r.buckettype) continue;
+        int nc = 0, npc = 0;
+        for (int ii = i - 1; ii < i + 2; ii++)
+          for (int jj = j - 1; jj < j + 2; jj++)
+            for (int kk = k - 1; kk < k + 2; kk++) {
+              if (ii > -1 && ii < nx && jj > -1 && jj < ny && kk > -1 && kk < nz) {
+                const BucketRecord &o = bg[B.idx(ii, jj, kk)];
+                for (int l = 0; l < kKeyLen; l++) r.neighs[nc++] = o.key[l];
+                r.neigh_proc[npc++] = (ii == i && jj == j && kk == k) ? -2 : o.myproc;
+              } else {
+                r.neighs[nc++] = 0;
+                r.neighs[nc++] = 0;
+                r.neigh_proc[npc++] = -1;
+              }
+            }
+      }
+  // partition order (PartiHead::operator<, preprocess.h:59-110): a bucket of
+  // the lowest layer sorts right after the bucket above it
+  struct Part { uint32_t k0, k1, u; size_t q; };
+  std::vector<Part> part;
+  part.reserve(B.size());
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        const size_t q = B.idx(i, j, k);
+        if (k == 0 && nz > 1) {
+          uint32_t top[kKeyLen];
+          key_at(i, j, 1, top);
+          part.push_back({top[0], top[1], 1u, q});
+        } else {
+          part.push_back({bg[q].key[0], bg[q].key[1], 0u, q});
+        }
+      }
+  std::stable_sort(part.begin(), part.end(), [](const Part &a, const Part &b) {
+    if (a.k0 != b.k0) return a.k0 < b.k0;
+    if (a.k1 != b.k1) return a.k1 < b.k1;
+    return a.u < b.u;
+  });
+  std::vector<BucketRecord> out;
+  std::vector<uint32_t> ptab;
+  for (const Part &p : part) {
+    const BucketRecord &r = bg[p.q];
+    if (r.buckettype > 0) out.push_back(r);
+    // the partition key is the bucket's own key (createfunky's table,
+    // preprocess.cc:341-349), then the bucket's key again
+    for (int l = 0; l < kKeyLen; l++) ptab.push_back(r.key[l]);
+    for (int l = 0; l < kKeyLen; l++) ptab.push_back(r.key[l]);
+  }
+  H5Id fp(H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT), H5Fclose);
+  if (fp < 0) return fail(PSPH_EIO, "grid file: cannot create %s", path);
+  const double ht[6] = {lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]};
+  PSPH_TRY(write_1d(fp, "/hashtable_constants", H5T_NATIVE_DOUBLE, ht, 6));
+  {
+    H5Id t(bucket_file_type(), H5Tclose);
+    PSPH_TRY(write_1d(fp, "/Buckets", t, out.data(), out.size()));
+  }
+  PSPH_TRY(write_1d(fp, "/partition_table", H5T_NATIVE_UINT, ptab.data(), ptab.size()));
+  return PSPH_OK;
 }

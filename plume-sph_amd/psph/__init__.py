@@ -29,6 +29,8 @@ BC_ERUPT_GHOST, BC_PRESSURE_GHOST, BC_WALL_GHOST, BC_NOSAVE_REAL, BC_REAL = 0, 1
 
 ERRORS = {0: "OK", -1: "EINVAL", -2: "ENOMEM", -3: "EINVARIANT", -4: "ECAPACITY",
           -5: "EHIP", -6: "EIO", -7: "ENOTSUP"}
+(PSPH_OK, PSPH_EINVAL, PSPH_ENOMEM, PSPH_EINVARIANT, PSPH_ECAPACITY, PSPH_EHIP, PSPH_EIO,
+ PSPH_ENOTSUP) = 0, -1, -2, -3, -4, -5, -6, -7
 
 
 class PsphError(RuntimeError):

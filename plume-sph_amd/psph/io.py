@@ -46,6 +46,9 @@ def io_lib():
                                        C.POINTER(C.c_size_t)]),
         "psph_csv_write_soa": (C.c_int, [C.c_char_p, SOA, C.c_size_t]),
         "psph_write_csv": (C.c_int, [P, C.c_char_p]),
+        "psph_grid_read": (C.c_int, [C.c_char_p, C.c_void_p]),
+        "psph_grid_check": (C.c_int, [C.c_char_p, C.c_void_p]),
+        "psph_grid_write": (C.c_int, [C.c_char_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -105,3 +108,27 @@ class H5PartWriter:
         check(io_lib().psph_write_h5part(solver._ctx, os.fsencode(self.path), int(self.count == 0), self.count,
                                          self.which, self.extras))
         self.count += 1
+
+
+class GridInfo(C.Structure):
+    """psph_grid_info: what a grid file (funky%04d.h5) holds."""
+    _fields_ = [("ext_lo", C.c_double * 3), ("ext_hi", C.c_double * 3), ("bucket_edge", C.c_double),
+                ("nb", C.c_int32 * 3), ("n_buckets", C.c_int64), ("n_type", C.c_int64 * 5),
+                ("nproc", C.c_int32), ("flat", C.c_int32), ("ground", C.c_double), ("n_partition", C.c_int64)]
+
+
+def grid_read(path: str) -> GridInfo:
+    """The extended domain, bucket grid, types and ground of a grid file."""
+    g = GridInfo()
+    check(io_lib().psph_grid_read(path.encode(), C.byref(g)))
+    return g
+
+
+def grid_check(path: str, cfg) -> None:
+    """Raises PsphError unless the grid file describes cfg's domain and buckets."""
+    check(io_lib().psph_grid_check(path.encode(), C.byref(cfg)))
+
+
+def grid_write(path: str, cfg) -> None:
+    """The one-process grid file of cfg's domain (flat ground)."""
+    check(io_lib().psph_grid_write(path.encode(), C.byref(cfg)))

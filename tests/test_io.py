@@ -186,3 +186,138 @@ def test_csv_format(tmp_path):
         exe = tmp_path / "row"
         subprocess.run([gxx, str(src), "-o", str(exe)], check=True)
         assert subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout == want
+
+
+# ------------------------------------------------------------ grid files
+# GH5_bucketstruct (hdf5calls.cc:175-203): scalar members at the offsets of
+# BucketStruct (buckstr.h:14-26, KEYLENGTH 2, NEIGH_SIZE 27)
+BUCKET_MEMBERS = [("Key", "H5T_STD_U32LE", 0), ("neighs", "H5T_STD_U32LE", 8),
+                  ("buckettype", "H5T_STD_I32LE", 224), ("myproc", "H5T_STD_I32LE", 228),
+                  ("neigh_proc", "H5T_STD_I32LE", 232), ("xcoord", "H5T_IEEE_F64LE", 368),
+                  ("ycoord", "H5T_IEEE_F64LE", 384), ("zcoord", "H5T_IEEE_F64LE", 400),
+                  ("elev", "H5T_IEEE_F64LE", 416)]
+RECORD = 448
+
+
+def grid_layout(path):
+    out = subprocess.run([H5DUMP, "-p", "-H", path], capture_output=True, text=True, check=True).stdout
+    ds = {}
+    for name, body in re.findall(r'DATASET "([^"]+)" \{(.*?)\n   \}', out, flags=re.S):
+        n = int(re.search(r"SIMPLE \{ \( (\d+) \)", body).group(1))
+        size = int(re.search(r"SIZE (\d+)", body).group(1))
+        off = int(re.search(r"OFFSET (\d+)", body).group(1))
+        members = re.findall(r'(H5T_\w+) "(\w+)";', body)
+        ds[name] = (n, size, off, [(m, t) for t, m in members])
+    return ds
+
+
+def grid_case(tmp_path):
+    from helpers import small_plume_config
+    cfg = small_plume_config()
+    path = str(tmp_path / "funky0000.h5")
+    pio.grid_write(path, cfg)
+    return cfg, path
+
+
+@needs_h5dump
+def test_grid_file_layout(tmp_path):
+    """The preprocessor's file: three datasets, the compound record of
+    GH5_bucketstruct (member names and types in its order, 448-byte records),
+    and each member's value at BucketStruct's offset in the raw bytes."""
+    import oracle
+    cfg, path = grid_case(tmp_path)
+    L = grid_layout(path)
+    assert set(L) == {"Buckets", "hashtable_constants", "partition_table"}
+    n, size, off, members = L["Buckets"]
+    assert [(m, t) for m, t, _ in BUCKET_MEMBERS] == members
+    assert size == n * RECORD
+    nb = [int(round((cfg.ext_hi[k] - cfg.ext_lo[k]) / cfg.bucket_edge)) for k in range(3)]
+    assert n == nb[0] * nb[1] * nb[2]
+    ht = raw(path, L["hashtable_constants"][2], 6, False)
+    assert list(ht) == [cfg.ext_lo[0], cfg.ext_hi[0], cfg.ext_lo[1], cfg.ext_hi[1], cfg.ext_lo[2], cfg.ext_hi[2]]
+    pn, psize, poff, _ = L["partition_table"]
+    assert pn == 4 * n and psize == 16 * n
+    ptab = np.frombuffer(open(path, "rb").read()[poff:poff + psize], "<u4").reshape(n, 4)
+    assert np.array_equal(ptab[:, :2], ptab[:, 2:])
+    blob = open(path, "rb").read()[off:off + size]
+
+    def member(r, name):
+        o = r * RECORD + dict((m, o) for m, _, o in BUCKET_MEMBERS)[name]
+        if name in ("Key", "neighs"):
+            return int(np.frombuffer(blob[o:o + 4], "<u4")[0])
+        if name in ("buckettype", "myproc", "neigh_proc"):
+            return int(np.frombuffer(blob[o:o + 4], "<i4")[0])
+        return float(np.frombuffer(blob[o:o + 8], "<f8")[0])
+
+    cell = lambda v, k: int(round((v - cfg.ext_lo[k]) / cfg.bucket_edge))
+    keys_by_cell = {}
+    for r in range(n):
+        b = (cell(member(r, "xcoord"), 0), cell(member(r, "ycoord"), 1), cell(member(r, "zcoord"), 2))
+        # type: the oracle's restatement of determine_bucket_type (preprocess.cc:380-440)
+        assert member(r, "buckettype") == oracle.bucket_type(cfg, *b)[0], b
+        assert member(r, "myproc") == 0 and member(r, "elev") == cfg.dom_lo[2]
+        # the key of the bucket centre (HSFC3d of the centre normalised to the hash-table domain)
+        c = [cfg.ext_lo[k] + (b[k] + 0.5) * cfg.bucket_edge for k in range(3)]
+        k0, _, _ = psph.keys(cfg, np.array([c[0]]), np.array([c[1]]), np.array([c[2]]))
+        assert member(r, "Key") == int(k0[0]), b
+        assert member(r, "Key") == int(ptab[r, 0])
+        keys_by_cell[b] = member(r, "Key")
+        # first neighbour (i-1, j-1, k-1): outside the grid -> key 0, process -1
+        inside = all(v > 0 for v in b)
+        assert member(r, "neigh_proc") == (0 if inside else -1)
+        if not inside:
+            assert member(r, "neighs") == 0
+    for r in range(n):
+        b = (cell(member(r, "xcoord"), 0), cell(member(r, "ycoord"), 1), cell(member(r, "zcoord"), 2))
+        if all(v > 0 for v in b):
+            assert member(r, "neighs") == keys_by_cell[(b[0] - 1, b[1] - 1, b[2] - 1)]
+    # partition order (PartiHead::operator<): by key, a lowest-layer bucket by
+    # the key of the bucket above it and right after that bucket
+    order = []
+    for r in range(n):
+        b = (cell(member(r, "xcoord"), 0), cell(member(r, "ycoord"), 1), cell(member(r, "zcoord"), 2))
+        if b[2] == 0:
+            top = keys_by_cell[(b[0], b[1], 1)]
+            order.append((top, 1))
+        else:
+            order.append((member(r, "Key"), 0))
+    assert [o[0] for o in order] == sorted(o[0] for o in order)
+
+
+def test_grid_file_read_and_check(tmp_path):
+    cfg, path = grid_case(tmp_path)
+    g = pio.grid_read(path)
+    assert list(g.ext_lo) == list(cfg.ext_lo) and list(g.ext_hi) == list(cfg.ext_hi)
+    assert g.bucket_edge == cfg.bucket_edge and g.nproc == 1 and g.flat == 1 and g.ground == cfg.dom_lo[2]
+    nb = [int(round((cfg.ext_hi[k] - cfg.ext_lo[k]) / cfg.bucket_edge)) for k in range(3)]
+    assert list(g.nb) == nb and g.n_buckets == nb[0] * nb[1] * nb[2] and g.n_partition == 4 * g.n_buckets
+    assert sum(g.n_type) == g.n_buckets and g.n_type[0] == 0
+    pio.grid_check(path, cfg)
+    # another deck's domain is refused
+    from helpers import small_plume_config
+    other = small_plume_config(half=1000.0)
+    with pytest.raises(psph.PsphError) as e:
+        pio.grid_check(path, other)
+    assert e.value.code == psph.PSPH_EINVAL
+
+
+def test_grid_file_terrain_and_processes_refused(tmp_path):
+    """A varying ground elevation (the preprocessor's GIS mode) or buckets of
+    another process: the file is read, the check refuses it (PSPH_ENOTSUP)."""
+    cfg, path = grid_case(tmp_path)
+    if not os.path.exists(H5DUMP):
+        pytest.skip("h5dump not in this image")
+    n, size, off, _ = grid_layout(path)["Buckets"]
+    data = bytearray(open(path, "rb").read())
+    for field, o, value, fmt in (("elev", 416, 12.5, "<f8"), ("myproc", 228, 1, "<i4")):
+        bad = bytearray(data)
+        at = off + 3 * RECORD + o
+        v = np.array([value], fmt).tobytes()
+        bad[at:at + len(v)] = v
+        p = str(tmp_path / f"bad_{field}.h5")
+        open(p, "wb").write(bytes(bad))
+        g = pio.grid_read(p)
+        assert (g.flat == 0) if field == "elev" else (g.nproc == 2)
+        with pytest.raises(psph.PsphError) as e:
+            pio.grid_check(p, cfg)
+        assert e.value.code == psph.PSPH_ENOTSUP, field
