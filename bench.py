@@ -313,7 +313,7 @@ def adjoint_leg(args):
     S.load(P)
     r_short = S.adjoint_run(n_short, n_short, grads=True)
     dirs_short = check(n_short, r_short, fd2, args.adjoint_eps, "two-point")
-    S.set_vent(theta0[0], theta0[1], 1.0 - theta0[2])
+    S.set_vent(cfg.vent_v, cfg.vent_T, cfg.ng0)
     S.load(P)
     r = S.adjoint_run(N, K, grads=True)
     log(f"[bench] adjoint: {len(P)} particles, {N} steps, ckpt {K}: forward {fwd_ms:.0f} ms, "
@@ -327,7 +327,7 @@ def adjoint_leg(args):
             fdv = fd2(N, eps, dth, None)
             sweep.append({"eps": eps, "adjoint": ad, "fd": fdv, "rel_err": abs(ad - fdv) / abs(fdv)})
             log(f"[bench] adjoint FD sweep vent_v eps {eps:g}: AD {ad:.9e} FD {fdv:.9e} rel {sweep[-1]['rel_err']:.2e}")
-    S.set_vent(theta0[0], theta0[1], 1.0 - theta0[2])
+    S.set_vent(cfg.vent_v, cfg.vent_T, cfg.ng0)
     S.close()
     return {"workload": f"weak plume, {args.adjoint_particles / 1e3:g}k real particles, {N} steps, "
                         f"checkpoint every {K}",

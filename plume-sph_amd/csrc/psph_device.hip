@@ -2645,6 +2645,19 @@ static int h_by_id(psph_ctx *c, double *tab, int replay) {
   return 0;
 }
 
+// A pass writes its outputs for its targets (need_neigh) into spare arrays;
+// the other particles keep their values: one launch copies those.
+struct FieldCopy {
+  const double *src[9];
+  double *dst[9];
+  int nf;
+};
+__global__ void k_copy_nontargets(uint32_t n, const int32_t *bc, FieldCopy fc) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || need_neigh(bc[i])) return;
+  for (int q = 0; q < fc.nf; q++) fc.dst[q][i] = fc.src[q][i];
+}
+
 extern "C" int psph_rhs(psph_ctx *c, double dt) {
   if (!c) return fail(PSPH_EINVAL, "null ctx");
   if (!c->grid_valid) return fail(PSPH_EINVAL, "rhs: call psph_neighbors first");
@@ -2653,8 +2666,12 @@ extern "C" int psph_rhs(psph_ctx *c, double dt) {
   // outputs go to spare arrays; non-targets keep their values
   double *ov[9] = {c->dsp[1], c->dsp[2], c->dsp[3], c->dsp[4], c->dsp[5], c->dsp[6], c->dsp[7], c->dsp[8], c->dsp[9]};
   double *src[9] = {c->f.vx, c->f.vy, c->f.vz, c->f.e, c->f.p, c->f.c, c->f.T, c->f.gam, c->f.cp};
-  for (int q = 0; q < 9; q++)
-    if (n) HIP_TRY(hipMemcpyAsync(ov[q], src[q], (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+  if (n) {
+    FieldCopy fc{};
+    for (int q = 0; q < 9; q++) { fc.src[q] = src[q]; fc.dst[q] = ov[q]; }
+    fc.nf = 9;
+    hipLaunchKernelGGL(k_copy_nontargets, blocks(n), 256, 0, c->stream, n, c->f.bc, fc);
+  }
   RhsArgs a{};
   a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.p = c->f.p; a.c = c->f.c; a.T = c->f.T; a.cp = c->f.cp;
   a.vx = c->f.vx; a.vy = c->f.vy; a.vz = c->f.vz; a.svx = c->f.svx; a.svy = c->f.svy; a.svz = c->f.svz;
@@ -2685,8 +2702,11 @@ extern "C" int psph_density(psph_ctx *c) {
   const psph_config &k = c->cfg;
   uint32_t n = (uint32_t)c->n;
   if (n) {
-    HIP_TRY(hipMemcpyAsync(c->dsp[1], c->f.rho, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->dsp[2], c->f.e, (size_t)n * 8, hipMemcpyDeviceToDevice, c->stream));
+    FieldCopy fc{};
+    fc.src[0] = c->f.rho; fc.dst[0] = c->dsp[1];
+    fc.src[1] = c->f.e; fc.dst[1] = c->dsp[2];
+    fc.nf = 2;
+    hipLaunchKernelGGL(k_copy_nontargets, blocks(n), 256, 0, c->stream, n, c->f.bc, fc);
   }
   DensArgs a{};
   a.h = c->f.h; a.m = c->f.m; a.rho = c->f.rho; a.e = c->f.e; a.bc = c->f.bc; a.phase = c->f.phase;
