@@ -618,6 +618,13 @@ struct VisitTgt {
   double ch, lf, supp, wide; // cutx h, list box, in_support box, widest union box
 };
 
+// v = value in lane `lane` only (value and lane wave-uniform): one
+// v_writelane_b32, which ignores EXEC.  With the value in a scalar register
+// the lane select goes through M0 (one scalar operand per VALU instruction).
+__device__ inline void set_lane(uint32_t &v, uint32_t value, uint32_t lane) {
+  asm volatile("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(value), "s"(lane) : "m0");
+}
+
 // FAST: for every target of the chunk the widened box lies inside both its
 // list box (4.25 h at list time) and its in_support box (CUTOFF2 h), and is
 // narrower than a bucket (so it reaches the adjacent buckets only): those
@@ -663,8 +670,6 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
   __syncthreads();
   const GridDesc &g = pc.tr.g;
   const uint32_t cq = 2u * ((uint32_t)lane & 31u) + ((uint32_t)lane >> 5);   // this lane's candidate
-  const uint32_t mine = (uint32_t)lane >> 1;                                 // this lane's target slot
-  const uint32_t hsh = 32u * ((uint32_t)lane & 1u);                          // its half of a ballot
   const uint32_t w0 = voff[chunk];
   uint32_t w = w0;
   for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
@@ -695,7 +700,7 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
           auto scan = [&](auto fast_c) {
             while (tm) {
               const uint32_t t = (uint32_t)__builtin_ctzll(tm);
-              tm &= tm - 1;
+              tm &= ~(1ull << t);
               const VisitTgt T = tg[t];
               double amax = fabs(T.x - cxp);
               if (D == 3) amax = fmax(amax, fmax(fabs(T.y - cyp), fabs(T.z - czp)));
@@ -713,9 +718,11 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
                 in &= amax <= T.supp;
                 bal = __ballot(in);
               }
-              // lanes 2t and 2t+1 (target t's sub-lanes) take the two halves
-              const uint32_t half = (uint32_t)(bal >> hsh);
-              mask = mine == t ? half : mask;
+              // lanes 2t and 2t+1 (target t's sub-lanes) take the two halves:
+              // t and the ballot are wave-uniform (scalar registers), so each
+              // half is written into its one lane
+              set_lane(mask, (uint32_t)bal, 2u * t);
+              set_lane(mask, (uint32_t)(bal >> 32), 2u * t + 1u);
             }
           };
           if (fast) scan(std::true_type{}); else scan(std::false_type{});
@@ -984,9 +991,13 @@ struct RhsPass {
     if (a.lans) { s.vx[l] = (R)a.svx[j]; s.vy[l] = (R)a.svy[j]; s.vz[l] = (R)a.svz[j]; }
     else { s.vx[l] = (R)a.vx[j]; s.vy[l] = (R)a.vy[j]; s.vz[l] = (R)a.vz[j]; }
   }
-  // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2); j != i (dW(0) = 0)
+  // h_ij = (h_i + h_j)/2 (ME_UPDATE_SML 2).  The reference skips j == i; here
+  // the target meets itself once, with dx = 0 and v_ij = 0, and every term it
+  // adds is an exact zero (f dx, f v.r, no viscosity for v.r = 0, no heat for
+  // |v_ij . dx| = 0), so no test is spent on it; finish leaves it out of the
+  // pair count.
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return t.hh + s.hh[q]; }
-  __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t j, uint32_t i) { return j != i; }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t, uint32_t) { return true; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *dxd, double) {
@@ -1124,7 +1135,7 @@ struct RhsPass {
     double p, c, T, gam, cp = a.cp[i];
     eos_dev(a.eos, a.rho[i], ne, a.xi[i], p, c, T, gam, &cp);
     a.o_p[i] = p; a.o_c[i] = c; a.o_T[i] = T; a.o_gam[i] = gam; a.o_cp[i] = cp;
-    if (t.npairs) atomicAdd(a.pairs, (unsigned long long)t.npairs);
+    if (t.npairs > 1) atomicAdd(a.pairs, (unsigned long long)(t.npairs - 1));   // without the target itself
   }
 };
 
@@ -1149,8 +1160,10 @@ struct DensArgs {
 template <int D> constexpr double kNorm = kInvSqrtPi * kInvSqrtPi * kInvSqrtPi;
 template <> constexpr double kNorm<1> = kInvSqrtPi;
 
-// mr = m / rho of the candidate (the energy-smoothing weight's factor)
-template <bool FX> struct DensMixTile { double m64[kWave], mr64[kWave], e64[kWave], etab[wtab_size<FX>()]; };
+// mr = m / rho of the candidate (the energy-smoothing weight's factor); the
+// mixed form stages m twice, as m1 (phase 1, else 0) and m2 (phase 2, else
+// 0), so that a pair adds to both sums without selecting between them
+template <bool FX> struct DensMixTile { double m1[kWave], m2[kWave], mr64[kWave], e64[kWave], etab[wtab_size<FX>()]; };
 template <class R> struct DensTileR { R m[kWave], mr[kWave], e[kWave]; };
 
 // The FP32-pair form evaluates the density and energy-smoothing weights in
@@ -1184,10 +1197,16 @@ struct DensPass {
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
     s.hh[l] = 0.5 * a.h[j];
-    if constexpr (kMixed) { s.m64[l] = a.m[j]; s.mr64[l] = a.m[j] / a.rho[j]; s.e64[l] = a.e[j]; }
-    else { s.m[l] = (R)a.m[j]; s.mr[l] = (R)(a.m[j] / a.rho[j]); s.e[l] = (R)a.e[j]; }
-    int ph = a.phase[j];
-    s.ok[l] = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
+    const int ph = a.phase[j];
+    const int ok = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
+    s.ok[l] = ok;
+    if constexpr (kMixed) {
+      const double mj = a.m[j];
+      s.m1[l] = ok == 1 ? mj : 0.0; s.m2[l] = ok == 2 ? mj : 0.0;
+      s.mr64[l] = mj / a.rho[j]; s.e64[l] = a.e[j];
+    } else {
+      s.m[l] = (R)a.m[j]; s.mr[l] = (R)(a.m[j] / a.rho[j]); s.e[l] = (R)a.e[j];
+    }
   }
   // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
   // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
@@ -1207,8 +1226,10 @@ struct DensPass {
       const double expt = r2 * ih2;
       const double ihD = (D == 3) ? ih2 * ih : ih;
       const double ex = exp_neg_w<FX>(expt, s.etab);
-      const double wm = (kNorm<D> * ihD) * (ex * s.m64[q]);
-      if (s.ok[q] == 1) t.r1 += wm; else t.r2 += wm;
+      // one of m1, m2 is 0: that sum gains an exact zero
+      const double wk = (kNorm<D> * ihD) * ex;
+      t.r1 = fma(wk, s.m1[q], t.r1);
+      t.r2 = fma(wk, s.m2[q], t.r2);
       if (a.energy_smooth && !a.cut.outside(a.e_ratio * amax, hijd)) {
         const double ee = a.e_sq16 ? pow16(ex) : exp_neg_w<FX>(a.e_ratio * a.e_ratio * expt, s.etab);
         const double vr = (a.e_norm * ihD) * (ee * s.mr64[q]);
