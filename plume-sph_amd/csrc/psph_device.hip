@@ -1201,9 +1201,11 @@ struct DensPass {
     const int ok = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
     s.ok[l] = ok;
     if constexpr (kMixed) {
+      // a candidate that does not contribute (ok 0) stages zero masses: its
+      // terms are exact zeros, so the mixed form needs no test for it
       const double mj = a.m[j];
       s.m1[l] = ok == 1 ? mj : 0.0; s.m2[l] = ok == 2 ? mj : 0.0;
-      s.mr64[l] = mj / a.rho[j]; s.e64[l] = a.e[j];
+      s.mr64[l] = ok ? mj / a.rho[j] : 0.0; s.e64[l] = ok ? a.e[j] : 0.0;
     } else {
       s.m[l] = (R)a.m[j]; s.mr[l] = (R)(a.m[j] / a.rho[j]); s.e[l] = (R)a.e[j];
     }
@@ -1211,7 +1213,9 @@ struct DensPass {
   // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
   // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return t.hh + s.hh[q]; }
-  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) {
+    return kMixed || s.ok[q] != 0;
+  }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds, double amax) {
