@@ -1296,6 +1296,10 @@ struct VsmArgs {
   double *svx, *svy, *svz;
   double cutoff, cutoff2;
   double *o_wn;            // optional: the filter's normalisation sum (adjoint recompute)
+  // the FP32 forms skip the per-pair contr_vel test (a non-contributing
+  // candidate stages m / rho = 0, so its terms are exact zeros) unless the
+  // step is recorded, where the boundary-ghost marks follow the tested pairs
+  int ok_test;
 };
 
 // The FP32-pair form evaluates the filter weights in FP64 from the exp table
@@ -1339,17 +1343,23 @@ struct VsmPass {
   }
   __device__ static void stage(const Args &a, uint32_t j, Tile &s, int l) {
     const double v[3] = {a.vx[j], a.vy ? a.vy[j] : 0.0, a.vz ? a.vz[j] : 0.0};
+    const int ok = contr_vel(a.bc[j]);
     if constexpr (kMixed) {
-      s.mr64[l] = a.m[j] / a.rho[j];
+      s.mr64[l] = ok ? a.m[j] / a.rho[j] : 0.0;
       for (int k = 0; k < 3; k++) s.v64[k][l] = v[k];
     } else {
       s.mr[l] = (R)(a.m[j] / a.rho[j]);
       s.vx[l] = (R)v[0]; s.vy[l] = (R)v[1]; s.vz[l] = (R)v[2];
     }
-    s.ok[l] = contr_vel(a.bc[j]);
+    s.ok[l] = ok;
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
-  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) { return s.ok[q] != 0; }
+  __device__ static bool extra(const Args &a, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) {
+    if constexpr (kMixed) {
+      if (!a.ok_test) return true;
+    }
+    return s.ok[q] != 0;
+  }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds, double) {
@@ -2767,6 +2777,7 @@ extern "C" int psph_vsmooth(psph_ctx *c) {
   a.bc = c->f.bc; a.svx = c->f.svx; a.svy = k.dim == 3 ? c->f.svy : nullptr; a.svz = k.dim == 3 ? c->f.svz : nullptr;
   a.cutoff = k.cutoff; a.cutoff2 = k.cutoff2;
   a.o_wn = c->rec ? c->rec->d[RF_WNV] : nullptr;
+  a.ok_test = c->rec ? 1 : 0;
   if (c->n) PSPH_TRY(ensure_visits(c));
   {
     StageTimer tm(c, 5);
