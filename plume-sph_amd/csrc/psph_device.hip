@@ -1160,11 +1160,13 @@ struct DensArgs {
 template <int D> constexpr double kNorm = kInvSqrtPi * kInvSqrtPi * kInvSqrtPi;
 template <> constexpr double kNorm<1> = kInvSqrtPi;
 
-// mr = m / rho of the candidate (the energy-smoothing weight's factor); the
-// mixed form stages m twice, as m1 (phase 1, else 0) and m2 (phase 2, else
-// 0), so that a pair adds to both sums without selecting between them
+// mr = m / rho of the candidate (the energy-smoothing weight's factor).  m is
+// staged twice, as m1 (phase 1, else 0) and m2 (phase 2, else 0), so that a
+// pair adds to both sums without selecting between them; a candidate that
+// does not contribute (contr_dens false or another phase) stages zero masses
+// and mr = 0, so its terms are exact zeros and no pair tests for it.
 template <bool FX> struct DensMixTile { double m1[kWave], m2[kWave], mr64[kWave], e64[kWave], etab[wtab_size<FX>()]; };
-template <class R> struct DensTileR { R m[kWave], mr[kWave], e[kWave]; };
+template <class R> struct DensTileR { R m1[kWave], m2[kWave], mr[kWave], e[kWave]; };
 
 // The FP32-pair form evaluates the density and energy-smoothing weights in
 // FP64 with the exp table: the pressure gradient of the next step amplifies
@@ -1176,7 +1178,6 @@ struct DensPass {
   // the mixed form stages only the FP64 copies it reads
   struct Tile : std::conditional<kMixed, DensMixTile<FX>, DensTileR<R>>::type {
     double hh[kWave];       // h / 2
-    int ok[kWave];   // phase (1/2) if contr_dens, else 0
   };
   __device__ static void init(Tile &s, int lane) {
     if constexpr (kMixed) wtab_fill<FX>(s.etab, lane);
@@ -1199,23 +1200,20 @@ struct DensPass {
     s.hh[l] = 0.5 * a.h[j];
     const int ph = a.phase[j];
     const int ok = (contr_dens(a.pghost, a.bc[j]) && (ph == 1 || ph == 2)) ? ph : 0;
-    s.ok[l] = ok;
+    const double mj = a.m[j];
+    const double m1 = ok == 1 ? mj : 0.0, m2 = ok == 2 ? mj : 0.0;
+    const double mr = ok ? mj / a.rho[j] : 0.0, e = ok ? a.e[j] : 0.0;
     if constexpr (kMixed) {
-      // a candidate that does not contribute (ok 0) stages zero masses: its
-      // terms are exact zeros, so the mixed form needs no test for it
-      const double mj = a.m[j];
-      s.m1[l] = ok == 1 ? mj : 0.0; s.m2[l] = ok == 2 ? mj : 0.0;
-      s.mr64[l] = ok ? mj / a.rho[j] : 0.0; s.e64[l] = ok ? a.e[j] : 0.0;
+      s.m1[l] = m1; s.m2[l] = m2; s.mr64[l] = mr; s.e64[l] = e;
     } else {
-      s.m[l] = (R)a.m[j]; s.mr[l] = (R)(a.m[j] / a.rho[j]); s.e[l] = (R)a.e[j];
+      s.m1[l] = (R)m1; s.m2[l] = (R)m2; s.mr[l] = (R)mr; s.e[l] = (R)e;
     }
   }
-  // contr_dens + phase; kernel at h_ij (DENSITY_UPDATE_SML 11).  The
-  // energy-smoothing kernel (support 0.75 h_ij) lies inside that box.
+  // kernel at h_ij (DENSITY_UPDATE_SML 11); contr_dens and the phase are in
+  // the staged masses.  The energy-smoothing kernel (support 0.75 h_ij) lies
+  // inside that box.
   __device__ static double hker(const Args &, const Tgt &t, const Tile &s, uint32_t q) { return t.hh + s.hh[q]; }
-  __device__ static bool extra(const Args &, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) {
-    return kMixed || s.ok[q] != 0;
-  }
+  __device__ static bool extra(const Args &, const Tgt &, const Tile &, uint32_t, uint32_t, uint32_t) { return true; }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
                                 const double *ds, double amax) {
@@ -1243,7 +1241,6 @@ struct DensPass {
     } else {
     const R hij = (R)hijd;
     const R ih = rrcp<R>(hij);
-    const R mj = s.m[q];
     R expt = 0;
 #pragma unroll
     for (int k = 0; k < D; k++) {
@@ -1252,8 +1249,9 @@ struct DensPass {
     }
     const R ihD = (D == 3) ? ih * ih * ih : ih;
     const R ex = rexp<R>(-expt);
-    const R wm = (R(kNorm<D>) * ihD) * (ex * mj);
-    if (s.ok[q] == 1) t.r1 += (double)wm; else t.r2 += (double)wm;
+    const R wk = (R(kNorm<D>) * ihD) * ex;
+    t.r1 += (double)(wk * s.m1[q]);
+    t.r2 += (double)(wk * s.m2[q]);
     if (a.energy_smooth) {
       // weight(4 ds / h_ij, h_ij / 4) m_j / rho_j (HAVE_ENERGY_SMOOTH 1); exact FP64 box test
       if (!a.cut.outside(a.e_ratio * amax, hijd)) {
@@ -1296,9 +1294,9 @@ struct VsmArgs {
   double *svx, *svy, *svz;
   double cutoff, cutoff2;
   double *o_wn;            // optional: the filter's normalisation sum (adjoint recompute)
-  // the FP32 forms skip the per-pair contr_vel test (a non-contributing
-  // candidate stages m / rho = 0, so its terms are exact zeros) unless the
-  // step is recorded, where the boundary-ghost marks follow the tested pairs
+  // the per-pair contr_vel test is skipped (a non-contributing candidate
+  // stages m / rho = 0, so its terms are exact zeros) unless the step is
+  // recorded, where the boundary-ghost marks follow the tested pairs
   int ok_test;
 };
 
@@ -1348,17 +1346,14 @@ struct VsmPass {
       s.mr64[l] = ok ? a.m[j] / a.rho[j] : 0.0;
       for (int k = 0; k < 3; k++) s.v64[k][l] = v[k];
     } else {
-      s.mr[l] = (R)(a.m[j] / a.rho[j]);
+      s.mr[l] = ok ? (R)(a.m[j] / a.rho[j]) : R(0);
       s.vx[l] = (R)v[0]; s.vy[l] = (R)v[1]; s.vz[l] = (R)v[2];
     }
     s.ok[l] = ok;
   }
   __device__ static double hker(const Args &, const Tgt &t, const Tile &, uint32_t) { return t.hi; }
   __device__ static bool extra(const Args &a, const Tgt &, const Tile &s, uint32_t q, uint32_t, uint32_t) {
-    if constexpr (kMixed) {
-      if (!a.ok_test) return true;
-    }
-    return s.ok[q] != 0;
+    return !a.ok_test || s.ok[q] != 0;
   }
   template <int D>
   __device__ static void pair_d(const Args &a, Tgt &t, const Tile &s, uint32_t q, uint32_t, uint32_t,
