@@ -107,7 +107,10 @@ def strong_leg(args):
     # a step record holds 30 doubles and 6 words per particle of capacity; the
     # adjoint's own arrays, visit lists and checkpoints take much of the rest
     per_step = 264.0 * S.capacity
-    K = int(max(1, min(args.strong_adjoint, 0.45 * free / per_step)))
+    # the segment of step records takes at most 30 % of the free HBM: the
+    # reverse sweep's two-role visit masks (512 B per visit, ~100 GB at 16M)
+    # and the checkpoints need the rest
+    K = int(max(1, min(args.strong_adjoint, 0.30 * free / per_step)))
     if args.strong_ckpt > 0:
         K = min(K, args.strong_ckpt)
     log(f"[bench] strong adjoint: {free / 1e9:.0f} GB free, {per_step / 1e9:.1f} GB per step record, "
