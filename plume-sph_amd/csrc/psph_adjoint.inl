@@ -251,8 +251,6 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
   __syncthreads();
   const GridDesc &g = pc.tr.g;
   const uint32_t cq = 2u * ((uint32_t)lane & 31u) + ((uint32_t)lane >> 5);   // this lane's candidate
-  const uint32_t mine = (uint32_t)lane >> 1;                                 // this lane's row slot
-  const bool odd = (lane & 1) != 0;
   const uint32_t w0 = voff[chunk];
   uint32_t w = w0;
   for (int cz = box.c0[2]; cz <= box.c1[2]; cz++)
@@ -284,10 +282,10 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
           const bool meets = owner && dxl <= r_i && dyl <= r_i && dzl <= r_i;
           uint64_t tm = __ballot(meets);
           if (!tm) continue;
-          uint64_t mask = 0;
+          uint32_t mown = 0, mnbr = 0;   // this lane's halves of the two role ballots
           while (tm) {
             const uint32_t t = (uint32_t)__builtin_ctzll(tm);
-            tm &= tm - 1;
+            tm &= ~(1ull << t);
             const AdjVisitRow R = rw[t];
             const double amax = fmax(fabs(R.x - cxp), fmax(fabs(R.y - cyp), fabs(R.z - czp)));
             const bool same = (pc.list_all != 0) | (crb == R.refb);
@@ -299,11 +297,14 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
             // this lane's candidate sums, row t is its neighbour
             const bool nbr = nn_j & (same | ((amax <= lfj) & adj)) & (amax <= c2hj) & (amax <= cutx * fmax(hj, hij));
             const uint64_t bo = __ballot(own), bn = __ballot(nbr);
-            // lanes 2t and 2t+1 (row t's sub-lanes) take the two halves
-            const uint64_t ho = odd ? (bo >> 32) : (bo & 0xffffffffull);
-            const uint64_t hn = odd ? (bn >> 32) : (bn & 0xffffffffull);
-            mask = mine == t ? (ho | hn << 32) : mask;
+            // lanes 2t and 2t+1 (row t's sub-lanes) take the two halves of
+            // each ballot (one v_writelane each)
+            set_lane(mown, (uint32_t)bo, 2u * t);
+            set_lane(mown, (uint32_t)(bo >> 32), 2u * t + 1u);
+            set_lane(mnbr, (uint32_t)bn, 2u * t);
+            set_lane(mnbr, (uint32_t)(bn >> 32), 2u * t + 1u);
           }
+          const uint64_t mask = (uint64_t)mnbr << 32 | mown;
           if (__ballot(mask != 0)) {
             if (lane == 0) vent[w] = make_uint2(base, cnt);
             __builtin_nontemporal_store(mask, vmask + (size_t)w * kWave + lane);
