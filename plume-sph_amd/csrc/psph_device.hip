@@ -621,6 +621,8 @@ struct VisitTgt {
 // v = value in lane `lane` only (value and lane wave-uniform): one
 // v_writelane_b32, which ignores EXEC.  With the value in a scalar register
 // the lane select goes through M0 (one scalar operand per VALU instruction).
+// M0 is written by a scalar move, so the lane select has no VALU-write
+// hazard; the asm touches registers only (M0 declared clobbered).  lane < 64.
 __device__ inline void set_lane(uint32_t &v, uint32_t value, uint32_t lane) {
   asm volatile("s_mov_b32 m0, %2\n\tv_writelane_b32 %0, %1, m0" : "+v"(v) : "s"(value), "s"(lane) : "m0");
 }
