@@ -1576,22 +1576,26 @@ __device__ inline bool bucket_of(const BcondArgs &a, const double *p, int *b) {
   return ok;
 }
 
-// One wavefront per 64 consecutive particles (in cell order, so a wave's
-// ghosts and their images are compact): the wave stages the candidate cells'
+// One wavefront per kBcondSpan consecutive particles (in cell order, so a
+// wave's ghosts and their images are compact): the wave stages the candidate cells'
 // particles 64 at a time into LDS, with their bucket indices and 1/rho; each
 // lane holding an imaged ghost marks the staged candidates that reach its
 // image, then walks only those, in index order.
+// consecutive particles per wavefront: 32 (lanes 32-63 only stage and test
+// candidates) gives twice the waves to hide the staging latency behind;
+// 1.74 ms per refresh at 4M against 1.83 ms with 64 and 2.19 ms with 16
+constexpr uint32_t kBcondSpan = 32;
 __global__ __launch_bounds__(kWave) void k_bcond(uint32_t n, BcondArgs a) {
   __shared__ double sx[kWave], sy[kWave], sz[kWave], sh[kWave], smw[kWave], sir[kWave];
   __shared__ double svx[kWave], svy[kWave], svz[kWave], se[kWave], sxi[kWave];
   __shared__ double sqx[kWave], sqy[kWave], sqz[kWave];
   __shared__ int sbk[3][kWave];
   const int lane = threadIdx.x;
-  const uint32_t i = blockIdx.x * kWave + lane;
+  const uint32_t i = blockIdx.x * kBcondSpan + lane;
   double coord[3] = {0, 0, 0}, refc[3] = {0, 0, 0};
   int bi[3] = {0, 0, 0};
   bool act = false;
-  if (i < n && a.bc[i] == 2) {
+  if ((uint32_t)lane < kBcondSpan && i < n && a.bc[i] == 2) {
     coord[0] = a.x[i]; coord[1] = a.y[i]; coord[2] = a.z[i];
     int bg[3];
     if (bucket_of(a, coord, bg) && bg[2] == a.bzg) {
@@ -2934,7 +2938,7 @@ static int apply_bcond_grid(psph_ctx *c, double disp, double *wrec = nullptr, si
   a.px = qx ? qx : c->f.x; a.py = qx ? qy : c->f.y; a.pz = qx ? qz : c->f.z;
   const double hm = std::max(c->h_max, k.ratio_sml_dx * k.h_air);
   a.reach = k.cutoff * hm * (1 + 1e-9) + disp * (1 + 1e-9);
-  hipLaunchKernelGGL(k_bcond, blocks(c->n, kWave), kWave, 0, c->stream, (uint32_t)c->n, a);
+  hipLaunchKernelGGL(k_bcond, blocks(c->n, kBcondSpan), kWave, 0, c->stream, (uint32_t)c->n, a);
   HIP_TRY(hipGetLastError());
   return 0;
 }
