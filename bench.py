@@ -476,7 +476,8 @@ def main():
                            "one (1e-10 vs the CPU oracle on small cases). fp32-exp (the reported form) runs the "
                            "viscosity, heat and energy pair terms in FP32 and evaluates every pair weight's "
                            "exponential with the FP32 hardware instruction on an FP64 argument rounded once to FP32, "
-                           "h, 1/h, norms, pressure terms and sums FP64; fp32-pair evaluates the weights' exponential in FP64 "
+                           "h, 1/h, norms, pressure terms and sums FP64, except the density weights, FP32 "
+                           "throughout and summed in FP64; fp32-pair evaluates the weights' exponential in FP64 "
                            "(slower, ~100x closer to FP64); fp32-pure runs the RHS and density pair arithmetic "
                            "all in FP32 (per-particle factors too)"}
         for form in ("fp32-exp", "fp32-pair", "fp32-pure"):

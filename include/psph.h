@@ -57,7 +57,8 @@ extern "C" {
 #define PSPH_FP32_EXP    3   /* PSPH_FP32_PAIR with the pair weights'
                                 exponential in FP32 hardware (argument formed
                                 in FP64 and rounded once; h, 1/h, norms and
-                                sums FP64): about an FP32 ulp per weight      */
+                                sums FP64), and the density weights in FP32
+                                throughout, summed in FP64                    */
 
 /* ---- smoothing kernel -------------------------------------------------- */
 /* The reference has one kernel, a Gaussian truncated per axis at |s_k| = 3
