@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kWave) void k_adj_visit_masks(AdjCommon pc, double 
           uint32_t mown = 0, mnbr = 0;   // this lane's halves of the two role ballots
           while (tm) {
             const uint32_t t = (uint32_t)__builtin_ctzll(tm);
-            tm &= ~(1ull << t);
+            clear_bit_u(tm, t);
             const AdjVisitRow R = rw[t];
             const double amax = fmax(fabs(R.x - cxp), fmax(fabs(R.y - cyp), fabs(R.z - czp)));
             const bool same = (pc.list_all != 0) | (crb == R.refb);

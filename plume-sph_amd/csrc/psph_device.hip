@@ -618,6 +618,12 @@ struct VisitTgt {
   double ch, lf, supp, wide; // cutx h, list box, in_support box, widest union box
 };
 
+// m &= ~(1 << b) for a wave-uniform 64-bit mask in scalar registers: one
+// s_bitset0_b64 (the and-not of a shifted one takes two scalar instructions)
+__device__ inline void clear_bit_u(uint64_t &m, uint32_t b) {
+  asm volatile("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(b));
+}
+
 // v = value in lane `lane` only (value and lane wave-uniform): one
 // v_writelane_b32, which ignores EXEC.  With the value in a scalar register
 // the lane select goes through M0 (one scalar operand per VALU instruction).
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(kWave) void k_visit_masks(PairCommon pc, const doub
           auto scan = [&](auto fast_c) {
             while (tm) {
               const uint32_t t = (uint32_t)__builtin_ctzll(tm);
-              tm &= ~(1ull << t);
+              clear_bit_u(tm, t);
               const VisitTgt T = tg[t];
               double amax = fabs(T.x - cxp);
               if (D == 3) amax = fmax(amax, fmax(fabs(T.y - cyp), fabs(T.z - czp)));
